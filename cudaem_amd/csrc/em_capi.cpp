@@ -1,0 +1,957 @@
+// em_capi.cpp -- the C ABI (include/cudaem_amd.h): plans, specimen upload,
+// operator entry points and the fused multislice pipelines.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cudaem_amd.h"
+#include "em_host.hpp"
+#include "em_kernels.hpp"
+
+using namespace em;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kZEps = 1e-4;  // epsilon_rel of the slicer (float eps of misc_dev.cuh:272-273)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+struct GraphKey {
+  int kind, B, S;
+  bool operator<(const GraphKey& o) const {
+    return kind != o.kind ? kind < o.kind : (B != o.B ? B < o.B : S < o.S);
+  }
+};
+
+struct Graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  ~Graph() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+}  // namespace
+
+struct em_plan_s {
+  em_grid grid{};
+  em_beam beam{};
+  int max_batch = 1;
+  hipStream_t stream = nullptr;
+  Constants k{};
+  double vr_factor = 0, gx0 = 0, gy0 = 0;
+  GridDev gdev{};
+  int probe_group = 0;
+  std::string err;
+  // spline settings and element overrides
+  int nR = 128;
+  double r_min = 0.005, r_max = 3.0, tap = 0.85;
+  std::map<int, ElementCoefs> custom;
+  // specimen (host)
+  bool has_specimen = false;
+  std::vector<double> ax, ay, az, aocc;
+  std::vector<int> aZ;
+  std::vector<int> types;  // distinct Z, ascending
+  std::vector<Spline> splines;
+  std::vector<SliceRange> slices;
+  int nbx = 0, nby = 0;
+  float bin = 0;
+  // device
+  DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
+  DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, detout;
+  int vstore_slices = -1;
+  std::map<GraphKey, std::unique_ptr<Graph>> graphs;
+  hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
+  ~em_plan_s() {
+    graphs.clear();
+    if (gstream) (void)hipStreamDestroy(gstream);
+  }
+
+  em_status fail(em_status s, const std::string& m) {
+    err = m;
+    return s;
+  }
+  em_status hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return EM_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return EM_ERR_HIP;
+  }
+  long long N() const { return (long long)grid.nx * grid.ny; }
+};
+
+#define EM_TRY(expr)                 \
+  do {                               \
+    em_status _s = (expr);           \
+    if (_s != EM_OK) return _s;      \
+  } while (0)
+#define EM_HIP(plan, expr) EM_TRY((plan)->hip((expr), #expr))
+
+namespace {
+
+em_status upload_twiddles(em_plan p, DevBuf& buf, int n) {
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * kPi * (double)m / (double)n;
+    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  EM_HIP(p, buf.ensure(sizeof(float2) * n));
+  EM_HIP(p, hipMemcpy(buf.p, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  return EM_OK;
+}
+
+// Probes per workgroup: enough workgroups to fill 256 CUs several times over,
+// and as many probes as possible sharing one register-held t or P.
+int row_pb(const em_plan p, int B) {
+  const int blocks = p->grid.ny / row_lines_per_block(p->grid.nx);
+  const int target = 2048;
+  int groups = std::max(1, std::min(B, (target + blocks - 1) / blocks));
+  return (B + groups - 1) / groups;
+}
+int col_pb(const em_plan p, int B) {
+  const int blocks = col_ntiles(p->grid.nx, p->grid.ny);
+  const int target = 512;
+  int groups = std::max(1, std::min(B, (target + blocks - 1) / blocks));
+  return (B + groups - 1) / groups;
+}
+
+RowArgs row_args(em_plan p, const float2* in, float2* out, int B) {
+  RowArgs a{};
+  a.in = in;
+  a.out = out;
+  a.pstride = p->N();
+  a.nx = p->grid.nx;
+  a.ny = p->grid.ny;
+  a.nprobe = B;
+  a.pb = row_pb(p, B);
+  a.tw = p->tw_x.as<float2>();
+  a.wgt = 1.f;
+  return a;
+}
+
+ColArgs col_args(em_plan p, const float2* in, float2* out, int B) {
+  ColArgs a{};
+  a.in = in;
+  a.out = out;
+  a.pstride = p->N();
+  a.nx = p->grid.nx;
+  a.ny = p->grid.ny;
+  a.nprobe = B;
+  a.pb = col_pb(p, B);
+  a.P = p->P.as<float2>();
+  a.tw = p->tw_y.as<float2>();
+  a.grid = p->gdev;
+  return a;
+}
+
+em_status row(em_plan p, unsigned mode, const RowArgs& a) {
+  return p->hip(launch_row(mode, a, p->stream), "row pass");
+}
+em_status col(em_plan p, unsigned mode, const ColArgs& a) {
+  return p->hip(launch_col(mode, a, p->stream), "column pass");
+}
+
+em_status ensure_specimen_potential(em_plan p);
+
+// Fused forward multislice over nb probes at wave (real space in, Fourier out).
+em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
+                        float2* ckpt, long long ckpt_slice_stride, bool in_mixed_layout) {
+  for (int s = 0; s < S; ++s) {
+    RowArgs r = row_args(p, wave, wave, nb);
+    unsigned m = ROW_POST_FFT;
+    if (s > 0 || in_mixed_layout) m |= ROW_PRE_IFFT;
+    if (V) {
+      m |= ROW_MUL_V;
+      r.V = V + (long long)s * p->N();
+    } else {
+      m |= ROW_MUL_T;
+      r.t = t + (long long)s * p->N();
+    }
+    if (ckpt) {
+      m |= ROW_CKPT_STORE;
+      r.ckpt = ckpt + (long long)s * ckpt_slice_stride;
+    }
+    EM_TRY(row(p, m, r));
+    ColArgs c = col_args(p, wave, wave, nb);
+    EM_TRY(col(p, s < S - 1 ? (COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE) : (COL_PRE_FFT | COL_STORE), c));
+  }
+  return EM_OK;
+}
+
+int group_size(em_plan p, int B) {
+  if (p->probe_group > 0) return std::min(B, p->probe_group);
+  // keep a group's waves (8 B per pixel) within ~128 MiB so a slice's two
+  // passes can be served from the 256 MiB last-level cache
+  const long long bytes = p->N() * 8;
+  int g = (int)std::max<long long>(1, (128ll << 20) / bytes);
+  return std::min(B, g);
+}
+
+}  // namespace
+
+extern "C" {
+
+int em_fft_size_supported(int n) { return fft_size_supported(n) ? 1 : 0; }
+
+em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch, em_plan* out) {
+  if (!grid || !beam || !out || max_batch <= 0) return EM_ERR_INVALID;
+  *out = nullptr;
+  if (!fft_size_supported(grid->nx) || !fft_size_supported(grid->ny)) return EM_ERR_UNSUPPORTED;
+  if (!(grid->lx > 0) || !(grid->ly > 0) || !(beam->e0_kv > 0)) return EM_ERR_INVALID;
+  auto p = std::make_unique<em_plan_s>();
+  p->grid = *grid;
+  p->beam = *beam;
+  p->max_batch = max_batch;
+  p->k = beam_constants(beam->e0_kv);
+  p->vr_factor = p->k.sigma / std::cos(beam->theta);
+  p->gx0 = std::sin(beam->theta) * std::cos(beam->phi) / p->k.lambda;
+  p->gy0 = std::sin(beam->theta) * std::sin(beam->phi) / p->k.lambda;
+  p->gdev = make_grid_dev(*grid);
+  EM_HIP(p.get(), prepare_col_kernels(grid->ny));
+  EM_HIP(p.get(), hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
+  EM_TRY(upload_twiddles(p.get(), p->tw_x, grid->nx));
+  EM_TRY(upload_twiddles(p.get(), p->tw_y, grid->ny));
+  // Propagator, built once per plan (wave_function.cuh:42-44)
+  em_plan raw = p.get();
+  EM_HIP(raw, p->P.ensure(sizeof(float2) * p->N()));
+  if (std::fabs(grid->dz) <= 1e-5) {
+    EM_HIP(raw, launch_fill(p->P.as<float2>(), p->N(), make_float2(1.f, 0.f), nullptr));
+  } else {
+    const double w = -kPi * p->k.lambda * grid->dz;
+    EM_HIP(raw, launch_propagator(p->gdev, w, p->gx0, p->gy0, grid->bwl, p->P.as<float2>(), nullptr));
+  }
+  EM_HIP(raw, hipDeviceSynchronize());
+  *out = p.release();
+  return EM_OK;
+}
+
+em_status em_plan_destroy(em_plan plan) {
+  if (!plan) return EM_ERR_INVALID;
+  if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+  delete plan;
+  return EM_OK;
+}
+
+em_status em_plan_set_stream(em_plan plan, void* s) {
+  if (!plan) return EM_ERR_INVALID;
+  plan->stream = reinterpret_cast<hipStream_t>(s);
+  plan->graphs.clear();
+  return EM_OK;
+}
+
+em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double* gamma) {
+  if (!plan) return EM_ERR_INVALID;
+  if (lambda) *lambda = plan->k.lambda;
+  if (sigma) *sigma = plan->k.sigma;
+  if (gamma) *gamma = plan->k.gamma;
+  return EM_OK;
+}
+
+em_status em_plan_set_probe_group(em_plan plan, int g) {
+  if (!plan || g < 0) return EM_ERR_INVALID;
+  plan->probe_group = g;
+  plan->graphs.clear();
+  return EM_OK;
+}
+
+const char* em_plan_last_error(em_plan plan) { return plan ? plan->err.c_str() : "null plan"; }
+
+em_status em_set_element_coefs(em_plan plan, int Z, int nk, const double* a, const double* b, int ng,
+                               const double* c, const double* d) {
+  if (!plan || Z <= 0 || nk < 0 || ng < 0 || (nk && (!a || !b)) || (ng && (!c || !d)))
+    return EM_ERR_INVALID;
+  ElementCoefs k;
+  k.a.assign(a, a + nk);
+  k.b.assign(b, b + nk);
+  if (ng) {
+    k.c.assign(c, c + ng);
+    k.d.assign(d, d + ng);
+  }
+  plan->custom[Z] = k;
+  return EM_OK;
+}
+
+em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max, double tap_frac) {
+  if (!plan || nR < 4 || nR > 256 || !(r_min > 0) || !(r_max > r_min) || !(tap_frac > 0) || !(tap_frac < 1))
+    return EM_ERR_INVALID;
+  plan->nR = nR;
+  plan->r_min = r_min;
+  plan->r_max = r_max;
+  plan->tap = tap_frac;
+  return EM_OK;
+}
+
+em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n_slices) {
+  if (!p || !at || at->n < 0 || (at->n > 0 && (!at->x || !at->y || !at->z || !at->Z))) return EM_ERR_INVALID;
+  if (border < 0 || border > 2) return EM_ERR_INVALID;
+  const long long n = at->n;
+  // sort by z (stable): Slice_Generator requires z-sorted atoms (slice_generator.hpp:55)
+  std::vector<long long> order(n);
+  for (long long i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) { return at->z[a] < at->z[b]; });
+  p->ax.resize(n); p->ay.resize(n); p->az.resize(n); p->aocc.resize(n); p->aZ.resize(n);
+  for (long long i = 0; i < n; ++i) {
+    const long long j = order[i];
+    p->ax[i] = at->x[j];
+    p->ay[i] = at->y[j];
+    p->az[i] = at->z[j];
+    p->aZ[i] = at->Z[j];
+    p->aocc[i] = at->occ ? at->occ[j] : 1.0;
+    if (at->Z[j] <= 0 || at->Z[j] > 103) return p->fail(EM_ERR_INVALID, "atomic number out of range");
+  }
+  p->types = p->aZ;
+  std::sort(p->types.begin(), p->types.end());
+  p->types.erase(std::unique(p->types.begin(), p->types.end()), p->types.end());
+  if ((int)p->types.size() > 8) return p->fail(EM_ERR_UNSUPPORTED, "more than 8 elements");
+  p->splines.clear();
+  for (int Z : p->types) {
+    auto it = p->custom.find(Z);
+    const ElementCoefs k = it != p->custom.end() ? it->second : moliere_coefs(Z);
+    p->splines.push_back(build_spline(k, p->nR, p->r_min, p->r_max, p->tap));
+  }
+  p->slices = make_slices(p->az, p->grid.dz, border, kZEps);
+  const int S = (int)p->slices.size();
+  // bins of size >= R_max covering [-bin, lx + bin) x [-bin, ly + bin)
+  double rmax = 0;
+  for (auto& s : p->splines) rmax = std::max(rmax, std::sqrt(s.rmax2));
+  const double bin = std::max(rmax, 1e-3);
+  p->bin = (float)bin;
+  p->nbx = (int)std::ceil((p->grid.lx + 2 * bin) / bin);
+  p->nby = (int)std::ceil((p->grid.ly + 2 * bin) / bin);
+  const int nb = p->nbx * p->nby;
+  std::vector<float4> atoms_h;
+  std::vector<int> off_h((size_t)std::max(S, 1) * (nb + 1), 0);
+  atoms_h.reserve(n);
+  for (int s = 0; s < S; ++s) {
+    const SliceRange& r = p->slices[s];
+    std::vector<std::vector<long long>> bins(nb);
+    for (long long i = r.i0; i <= r.i1; ++i) {
+      int bx = (int)std::floor((p->ax[i] + bin) / bin), by = (int)std::floor((p->ay[i] + bin) / bin);
+      bx = std::min(std::max(bx, 0), p->nbx - 1);
+      by = std::min(std::max(by, 0), p->nby - 1);
+      bins[by * p->nbx + bx].push_back(i);
+    }
+    int* o = off_h.data() + (size_t)s * (nb + 1);
+    for (int bI = 0; bI < nb; ++bI) {
+      o[bI] = (int)atoms_h.size();
+      for (long long i : bins[bI]) {
+        const int t = (int)(std::lower_bound(p->types.begin(), p->types.end(), p->aZ[i]) - p->types.begin());
+        float4 a;
+        a.x = (float)p->ax[i];
+        a.y = (float)p->ay[i];
+        a.z = (float)(p->vr_factor * p->aocc[i]);
+        int ti = t;
+        std::memcpy(&a.w, &ti, sizeof(int));
+        atoms_h.push_back(a);
+      }
+    }
+    o[nb] = (int)atoms_h.size();
+  }
+  if (atoms_h.empty()) atoms_h.push_back(make_float4(0, 0, 0, 0));
+  const int nt = (int)p->types.size();
+  std::vector<float> tab_h((size_t)std::max(nt, 1) * 5 * p->nR, 0.f), rm_h(std::max(nt, 1), 0.f);
+  for (int t = 0; t < nt; ++t) {
+    const Spline& sp = p->splines[t];
+    float* tb = tab_h.data() + (size_t)t * 5 * p->nR;
+    for (int i = 0; i < p->nR; ++i) {
+      tb[i] = (float)sp.R2[i];
+      tb[p->nR + i] = (float)sp.c0[i];
+      tb[2 * p->nR + i] = (float)sp.c1[i];
+      tb[3 * p->nR + i] = (float)sp.c2[i];
+      tb[4 * p->nR + i] = (float)sp.c3[i];
+    }
+    rm_h[t] = (float)sp.rmax2;
+  }
+  EM_HIP(p, p->atoms.ensure(sizeof(float4) * atoms_h.size()));
+  EM_HIP(p, hipMemcpy(p->atoms.p, atoms_h.data(), sizeof(float4) * atoms_h.size(), hipMemcpyHostToDevice));
+  EM_HIP(p, p->off.ensure(sizeof(int) * off_h.size()));
+  EM_HIP(p, hipMemcpy(p->off.p, off_h.data(), sizeof(int) * off_h.size(), hipMemcpyHostToDevice));
+  EM_HIP(p, p->tables.ensure(sizeof(float) * tab_h.size()));
+  EM_HIP(p, hipMemcpy(p->tables.p, tab_h.data(), sizeof(float) * tab_h.size(), hipMemcpyHostToDevice));
+  EM_HIP(p, p->rmax2.ensure(sizeof(float) * rm_h.size()));
+  EM_HIP(p, hipMemcpy(p->rmax2.p, rm_h.data(), sizeof(float) * rm_h.size(), hipMemcpyHostToDevice));
+  p->has_specimen = true;
+  p->vstore_slices = -1;
+  p->graphs.clear();
+  if (n_slices) *n_slices = S;
+  return EM_OK;
+}
+
+em_status em_slice_info(em_plan p, int s, double* zs, double* ze, long long* i0, long long* i1) {
+  if (!p || !p->has_specimen || s < 0 || s >= (int)p->slices.size()) return EM_ERR_INVALID;
+  const SliceRange& r = p->slices[s];
+  if (zs) *zs = r.z_start;
+  if (ze) *ze = r.z_end;
+  if (i0) *i0 = r.i0;
+  if (i1) *i1 = r.i1;
+  return EM_OK;
+}
+
+em_status em_spline_tables(em_plan p, int max_types, int* Zs, int* ntypes, int* nR, double* R2,
+                           double* c0, double* c1, double* c2, double* c3) {
+  if (!p || !p->has_specimen) return EM_ERR_INVALID;
+  const int nt = (int)p->types.size();
+  if (ntypes) *ntypes = nt;
+  if (nR) *nR = p->nR;
+  if (max_types < nt) return EM_OK;  // sizes only
+  for (int t = 0; t < nt; ++t) {
+    if (Zs) Zs[t] = p->types[t];
+    const Spline& s = p->splines[t];
+    const size_t o = (size_t)t * p->nR;
+    for (int i = 0; i < p->nR; ++i) {
+      if (R2) R2[o + i] = s.R2[i];
+      if (c0) c0[o + i] = s.c0[i];
+      if (c1) c1[o + i] = s.c1[i];
+      if (c2) c2[o + i] = s.c2[i];
+      if (c3) c3[o + i] = s.c3[i];
+    }
+  }
+  return EM_OK;
+}
+
+em_status em_potential_slices(em_plan p, int s0, int ns, float* V) {
+  if (!p || !p->has_specimen || !V || s0 < 0 || ns <= 0 || s0 + ns > (int)p->slices.size())
+    return EM_ERR_INVALID;
+  const int nb = p->nbx * p->nby;
+  EM_HIP(p, launch_potential(p->gdev, ns, p->off.as<int>() + (size_t)s0 * (nb + 1), p->nbx, p->nby,
+                             p->bin, p->atoms.as<float4>(), (int)p->types.size(), p->nR,
+                             p->tables.as<float>(), p->rmax2.as<float>(), V, p->stream));
+  return EM_OK;
+}
+
+em_status em_propagator(em_plan p, void* P) {
+  if (!p || !P) return EM_ERR_INVALID;
+  EM_HIP(p, hipMemcpyAsync(P, p->P.p, sizeof(float2) * p->N(), hipMemcpyDeviceToDevice, p->stream));
+  return EM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Lens table A for the given lens/defocus, probe ramps for nb probes, then the
+// column pass that turns A * ramps into F_y^-1 of the Fourier-space probe
+// (the (y, kx) layout the row pass reads).
+em_status probes_mixed(em_plan p, const em_lens* cond, double defocus_shift, const double* x,
+                       const double* y, int nb, float2* wave) {
+  const long long N = p->N();
+  EM_HIP(p, p->A.ensure(sizeof(float2) * N));
+  EM_HIP(p, launch_lens_table(p->gdev, derive_lens(*cond, p->k.lambda, defocus_shift), p->A.as<float2>(), p->stream));
+  std::vector<double> xe(nb), ye(nb);
+  for (int b = 0; b < nb; ++b) {
+    xe[b] = -2.0 * kPi * (x[b] - 0.5 * p->grid.lx);
+    ye[b] = -2.0 * kPi * (y[b] - 0.5 * p->grid.ly);
+  }
+  EM_HIP(p, p->xe.ensure(sizeof(double) * nb));
+  EM_HIP(p, p->ye.ensure(sizeof(double) * nb));
+  EM_HIP(p, hipMemcpyAsync(p->xe.p, xe.data(), sizeof(double) * nb, hipMemcpyHostToDevice, p->stream));
+  EM_HIP(p, hipMemcpyAsync(p->ye.p, ye.data(), sizeof(double) * nb, hipMemcpyHostToDevice, p->stream));
+  EM_HIP(p, p->ex.ensure(sizeof(float2) * nb * p->grid.nx));
+  EM_HIP(p, p->ey.ensure(sizeof(float2) * nb * p->grid.ny));
+  EM_HIP(p, launch_probe_ramps(p->gdev, nb, p->xe.as<double>(), p->ye.as<double>(), p->ex.as<float2>(),
+                               p->ey.as<float2>(), p->stream));
+  ColArgs c = col_args(p, nullptr, wave, nb);
+  c.A = p->A.as<float2>();
+  c.ex = p->ex.as<float2>();
+  c.ey = p->ey.as<float2>();
+  EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
+  // the host vectors must outlive the asynchronous copies
+  EM_HIP(p, hipStreamSynchronize(p->stream));
+  return EM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+em_status em_build_probes(em_plan p, const em_lens* cond, double defocus_shift, const double* x,
+                          const double* y, int nprobe, void* psi) {
+  if (!p || !cond || !x || !y || !psi || nprobe <= 0) return EM_ERR_INVALID;
+  float2* w = reinterpret_cast<float2*>(psi);
+  EM_TRY(probes_mixed(p, cond, defocus_shift, x, y, nprobe, w));
+  RowArgs r = row_args(p, w, w, nprobe);
+  EM_TRY(row(p, ROW_PRE_IFFT, r));
+  return EM_OK;
+}
+
+em_status em_plane_wave(em_plan p, int nprobe, void* psi) {
+  if (!p || !psi || nprobe <= 0) return EM_ERR_INVALID;
+  EM_HIP(p, launch_fill(reinterpret_cast<float2*>(psi), p->N() * nprobe, make_float2(1.f, 0.f), p->stream));
+  return EM_OK;
+}
+
+em_status em_fft2d(em_plan p, void* data, int nbatch, int dir) {
+  if (!p || !data || nbatch <= 0 || (dir != -1 && dir != 1)) return EM_ERR_INVALID;
+  float2* w = reinterpret_cast<float2*>(data);
+  if (dir < 0) {
+    EM_TRY(row(p, ROW_POST_FFT, row_args(p, w, w, nbatch)));
+    EM_TRY(col(p, COL_PRE_FFT | COL_STORE, col_args(p, w, w, nbatch)));
+  } else {
+    EM_TRY(col(p, COL_PRE_IFFT | COL_STORE, col_args(p, w, w, nbatch)));
+    EM_TRY(row(p, ROW_PRE_IFFT, row_args(p, w, w, nbatch)));
+  }
+  return EM_OK;
+}
+
+em_status em_multislice_forward(em_plan p, const float* V, const void* t, int S, void* psi, int nprobe,
+                                void* ckpt) {
+  if (!p || !psi || S <= 0 || nprobe <= 0 || (!V) == (!t)) return EM_ERR_INVALID;
+  float2* w = reinterpret_cast<float2*>(psi);
+  const int G = group_size(p, nprobe);
+  for (int g0 = 0; g0 < nprobe; g0 += G) {
+    const int nb = std::min(G, nprobe - g0);
+    float2* ck = ckpt ? reinterpret_cast<float2*>(ckpt) + (long long)g0 * p->N() : nullptr;
+    EM_TRY(forward_group(p, V, reinterpret_cast<const float2*>(t), S, w + (long long)g0 * p->N(), nb, ck,
+                         (long long)nprobe * p->N(), false));
+  }
+  return EM_OK;
+}
+
+em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S, const void* ckpt,
+                                void* gPsi, int nprobe, void* gt, float* gV, void* gpsi0) {
+  if (!p || !ckpt || !gPsi || !gt || S <= 0 || nprobe <= 0 || (!V) == (!t)) return EM_ERR_INVALID;
+  const long long N = p->N();
+  float2* lam = reinterpret_cast<float2*>(gPsi);
+  const float2* ck = reinterpret_cast<const float2*>(ckpt);
+  const float2* tc = reinterpret_cast<const float2*>(t);
+  float2* gtc = reinterpret_cast<float2*>(gt);
+  const int pb = row_pb(p, nprobe);
+  const int nparts = (nprobe + pb - 1) / pb;
+  EM_HIP(p, p->gpart.ensure(sizeof(float2) * N * nparts));
+  // lambda_{S-1} = FFT^H gPsi: column inverse now, row inverse inside the first backward row pass
+  EM_TRY(col(p, COL_PRE_IFFT | COL_STORE, col_args(p, lam, lam, nprobe)));
+  for (int k = S - 1; k >= 0; --k) {
+    RowArgs r = row_args(p, lam, lam, nprobe);
+    r.ckpt = const_cast<float2*>(ck) + (long long)k * nprobe * N;
+    r.gpart = nparts == 1 ? gtc + (long long)k * N : p->gpart.as<float2>();
+    unsigned m = ROW_PRE_IFFT | ROW_GRAD;
+    if (k > 0 || gpsi0) {
+      m |= ROW_CONJ | (V ? ROW_MUL_V : ROW_MUL_T);
+      if (V) r.V = V + (long long)k * N; else r.t = tc + (long long)k * N;
+    }
+    if (k > 0) m |= ROW_POST_FFT;
+    else if (gpsi0) r.out = reinterpret_cast<float2*>(gpsi0);  // conj(t_0) lambda_0, real space
+    EM_TRY(row(p, m, r));
+    if (nparts > 1)
+      EM_HIP(p, launch_grad_reduce(N, nparts, p->gpart.as<float2>(), gtc + (long long)k * N, p->stream));
+    if (k > 0) EM_TRY(col(p, COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE, col_args(p, lam, lam, nprobe)));
+  }
+  if (V && gV) EM_HIP(p, launch_grad_v(gtc, V, 1.f, N * S, gV, p->stream));
+  return EM_OK;
+}
+
+em_status em_detector_sums(em_plan p, const void* Psi, int nprobe, int ndet, const double* inner,
+                           const double* outer, float w, int accumulate, float* out) {
+  if (!p || !Psi || !out || nprobe <= 0 || ndet <= 0 || ndet > kMaxDet || !inner || !outer)
+    return EM_ERR_INVALID;
+  const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+  EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)nprobe * ndet * ntiles));
+  ColArgs c = col_args(p, reinterpret_cast<const float2*>(Psi), nullptr, nprobe);
+  c.ndet = ndet;
+  for (int d = 0; d < ndet; ++d) {
+    c.dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+    c.dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
+  }
+  c.dpart = p->dpart.as<float>();
+  EM_TRY(col(p, COL_DET, c));
+  EM_HIP(p, launch_det_reduce(nprobe, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out, nprobe, p->stream));
+  return EM_OK;
+}
+
+em_status em_ctf_image(em_plan p, const em_lens* obj, const void* Psi, int nprobe, float* I, float* re,
+                       float* im, float w, int accumulate) {
+  if (!p || !obj || !Psi || nprobe <= 0 || (!I && !(re && im))) return EM_ERR_INVALID;
+  const long long N = p->N();
+  EM_HIP(p, p->A.ensure(sizeof(float2) * N));
+  EM_HIP(p, launch_lens_table(p->gdev, derive_lens(*obj, p->k.lambda, 0.0), p->A.as<float2>(), p->stream));
+  EM_HIP(p, p->tmp.ensure(sizeof(float2) * N * nprobe));
+  ColArgs c = col_args(p, reinterpret_cast<const float2*>(Psi), p->tmp.as<float2>(), nprobe);
+  c.A = p->A.as<float2>();
+  EM_TRY(col(p, COL_LOAD_MUL_A | COL_POST_IFFT | COL_STORE, c));
+  RowArgs r = row_args(p, p->tmp.as<float2>(), nullptr, nprobe);
+  r.wgt = w;
+  unsigned m = ROW_PRE_IFFT | (accumulate ? ROW_OUT_ACCUM : 0u);
+  if (I) {
+    r.o1 = I;
+    m |= ROW_OUT_ABS2;
+  } else {
+    r.o1 = re;
+    r.o2 = im;
+    m |= ROW_OUT_SPLIT;
+  }
+  EM_TRY(row(p, m, r));
+  return EM_OK;
+}
+
+em_status em_diffraction(em_plan p, const void* Psi, int nprobe, float* I, float w, int accumulate) {
+  if (!p || !Psi || !I || nprobe <= 0) return EM_ERR_INVALID;
+  const long long N = p->N();
+  EM_HIP(p, p->tmp.ensure(sizeof(float) * N * nprobe));
+  EM_HIP(p, launch_abs2(reinterpret_cast<const float2*>(Psi), p->tmp.as<float>(), N * nprobe, 1.f, 0, p->stream));
+  EM_HIP(p, launch_fft_shift(p->tmp.as<float>(), I, p->grid.nx, p->grid.ny, nprobe, w, accumulate, p->stream));
+  return EM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Sigma-scaled potentials of every slice of the current specimen, built once
+// and kept (the stored-slice cache of transmission_function.cuh:65-101).
+em_status ensure_specimen_potential(em_plan p) {
+  if (!p->has_specimen) return p->fail(EM_ERR_INVALID, "no specimen");
+  const int S = (int)p->slices.size();
+  if (S <= 0) return p->fail(EM_ERR_INVALID, "specimen has no slices");
+  if (p->vstore_slices == S && p->Vstore.p) return EM_OK;
+  EM_HIP(p, p->Vstore.ensure(sizeof(float) * p->N() * S));
+  p->vstore_slices = -1;
+  EM_TRY(em_potential_slices(p, 0, S, p->Vstore.as<float>()));
+  p->vstore_slices = S;
+  return EM_OK;
+}
+
+// The body of one STEM probe batch (scan_simulator.cuh:88-109): probes of the
+// batch (already staged in xe / ye) are processed in groups; each group builds
+// its probes, runs every slice and leaves detector partials in dpart.
+em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const double* dg2max, int ndet) {
+  const long long N = p->N();
+  const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+  EM_HIP(p, launch_probe_ramps(p->gdev, nb, p->xe.as<double>(), p->ye.as<double>(), p->ex.as<float2>(),
+                               p->ey.as<float2>(), p->stream));
+  const int G = group_size(p, nb);
+  float2* wave = p->wave.as<float2>();
+  for (int g0 = 0; g0 < nb; g0 += G) {
+    const int ng = std::min(G, nb - g0);
+    ColArgs c = col_args(p, nullptr, wave, ng);
+    c.A = p->A.as<float2>();
+    c.ex = p->ex.as<float2>() + (long long)g0 * p->grid.nx;
+    c.ey = p->ey.as<float2>() + (long long)g0 * p->grid.ny;
+    EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
+    for (int s = 0; s < S; ++s) {
+      RowArgs r = row_args(p, wave, wave, ng);
+      r.V = p->Vstore.as<float>() + (long long)s * N;
+      EM_TRY(row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r));
+      ColArgs cc = col_args(p, wave, wave, ng);
+      if (s < S - 1) {
+        EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc));
+      } else {
+        cc.ndet = ndet;
+        for (int d = 0; d < ndet; ++d) {
+          cc.dg2min[d] = dg2min[d];
+          cc.dg2max[d] = dg2max[d];
+        }
+        cc.dpart = p->dpart.as<float>() + (long long)g0 * ndet * ntiles;
+        EM_TRY(col(p, COL_PRE_FFT | COL_DET, cc));
+      }
+    }
+  }
+  return EM_OK;
+}
+
+struct StreamSwap {
+  em_plan p;
+  hipStream_t saved;
+  StreamSwap(em_plan p_, hipStream_t s) : p(p_), saved(p_->stream) { p->stream = s; }
+  ~StreamSwap() { p->stream = saved; }
+};
+
+}  // namespace
+
+extern "C" {
+
+em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, const double* y, int ndet,
+                  const double* inner, const double* outer, float w, int accumulate, float* out) {
+  if (!p || !cond || !x || !y || !out || npos <= 0 || ndet <= 0 || ndet > kMaxDet || !inner || !outer)
+    return EM_ERR_INVALID;
+  EM_TRY(ensure_specimen_potential(p));
+  const int S = (int)p->slices.size();
+  const long long N = p->N();
+  const int B = std::min(p->max_batch, npos);
+  const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+  double dg2min[kMaxDet], dg2max[kMaxDet];
+  for (int d = 0; d < ndet; ++d) {
+    dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+    dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
+  }
+  // every buffer the graph touches is sized before capture
+  EM_HIP(p, p->wave.ensure(sizeof(float2) * N * B));
+  EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)B * ndet * ntiles));
+  EM_HIP(p, p->A.ensure(sizeof(float2) * N));
+  EM_HIP(p, p->xe.ensure(sizeof(double) * B));
+  EM_HIP(p, p->ye.ensure(sizeof(double) * B));
+  EM_HIP(p, p->ex.ensure(sizeof(float2) * B * p->grid.nx));
+  EM_HIP(p, p->ey.ensure(sizeof(float2) * B * p->grid.ny));
+  // order the private graph stream after the caller's stream
+  hipEvent_t ev0, ev1;
+  EM_HIP(p, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+  EM_HIP(p, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  EM_HIP(p, hipEventRecord(ev0, p->stream));
+  EM_HIP(p, hipStreamWaitEvent(p->gstream, ev0, 0));
+  const hipStream_t user = p->stream;
+  em_status st = EM_OK;
+  {
+    StreamSwap sw(p, p->gstream);
+    // condenser-lens table with the defocus moved to the middle of slice 0
+    // (incident_wave.cuh:33-35)
+    const double zmid0 = 0.5 * (p->slices[0].z_start + p->slices[0].z_end);
+    st = p->hip(launch_lens_table(p->gdev, derive_lens(*cond, p->k.lambda, zmid0), p->A.as<float2>(), p->stream),
+                "lens table");
+    std::map<int, std::unique_ptr<Graph>> graphs;  // per batch size, this call only
+    std::vector<double> xe(B), ye(B);
+    for (int start = 0; start < npos && st == EM_OK; start += B) {
+      const int nb = std::min(B, npos - start);
+      for (int b = 0; b < nb; ++b) {
+        xe[b] = -2.0 * kPi * (x[start + b] - 0.5 * p->grid.lx);
+        ye[b] = -2.0 * kPi * (y[start + b] - 0.5 * p->grid.ly);
+      }
+      // the previous batch has finished reading xe / ye once the stream drains
+      if ((st = p->hip(hipStreamSynchronize(p->stream), "sync")) != EM_OK) break;
+      if ((st = p->hip(hipMemcpy(p->xe.p, xe.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "xe")) != EM_OK) break;
+      if ((st = p->hip(hipMemcpy(p->ye.p, ye.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "ye")) != EM_OK) break;
+      auto it = graphs.find(nb);
+      if (it == graphs.end()) {
+        auto g = std::make_unique<Graph>();
+        if ((st = p->hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture")) != EM_OK) break;
+        const em_status body = stem_batch_body(p, nb, S, dg2min, dg2max, ndet);
+        const hipError_t ec = hipStreamEndCapture(p->stream, &g->g);
+        if (body != EM_OK) { st = body; break; }
+        if ((st = p->hip(ec, "end capture")) != EM_OK) break;
+        if ((st = p->hip(hipGraphInstantiate(&g->x, g->g, nullptr, nullptr, 0), "instantiate")) != EM_OK) break;
+        it = graphs.emplace(nb, std::move(g)).first;
+      }
+      if ((st = p->hip(hipGraphLaunch(it->second->x, p->stream), "graph launch")) != EM_OK) break;
+      st = p->hip(launch_det_reduce(nb, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out + start, npos, p->stream),
+                  "detector reduce");
+    }
+    if (st == EM_OK) st = p->hip(hipStreamSynchronize(p->stream), "sync");
+  }
+  (void)hipEventRecord(ev1, p->gstream);
+  (void)hipStreamWaitEvent(user, ev1, 0);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  return st;
+}
+
+em_status em_thickness(em_plan p, const float* V, int S, void* psi, int nprobe, double inner, double outer,
+                       int stride, float w, int accumulate, float* frames, int* n_frames) {
+  if (!p || !V || !psi || S <= 0 || nprobe <= 0 || !frames) return EM_ERR_INVALID;
+  const long long N = p->N();
+  float2* wv = reinterpret_cast<float2*>(psi);
+  EM_HIP(p, p->tmp.ensure(sizeof(float2) * N * nprobe));
+  int nf = 0;
+  for (int s = 0; s < S; ++s) {
+    RowArgs r = row_args(p, wv, wv, nprobe);
+    r.V = V + (long long)s * N;
+    EM_TRY(row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, r));
+    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, wv, wv, nprobe)));
+    const bool rec = stride <= 1 || ((s + 1) % stride == 0) || s == S - 1;
+    if (!rec) continue;
+    ColArgs c = col_args(p, wv, p->tmp.as<float2>(), nprobe);
+    c.fg2min = mrad_to_g2(inner, p->k.lambda, false);
+    c.fg2max = mrad_to_g2(outer, p->k.lambda, true);
+    c.fscale = (float)(1.0 / (double)N);
+    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_FILTER | COL_POST_IFFT | COL_STORE, c));
+    RowArgs ro = row_args(p, p->tmp.as<float2>(), nullptr, nprobe);
+    ro.o1 = frames + (long long)nf * nprobe * N;
+    ro.wgt = w;
+    EM_TRY(row(p, ROW_PRE_IFFT | ROW_OUT_ABS2 | (accumulate ? ROW_OUT_ACCUM : 0u), ro));
+    ++nf;
+  }
+  // back to real space
+  EM_TRY(row(p, ROW_PRE_IFFT, row_args(p, wv, wv, nprobe)));
+  if (n_frames) *n_frames = nf;
+  return EM_OK;
+}
+
+em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, const int* S, void* const* psi,
+                               const int* nprobe, int ndet, const double* inner, const double* outer, int stride,
+                               float* const* out, void* hip_stream) {
+  if (nreq <= 0 || !plans || !V || !S || !psi || !nprobe || !out || ndet <= 0 || ndet > kMaxDet || !inner ||
+      !outer || stride <= 0)
+    return EM_ERR_INVALID;
+  for (int r = 0; r < nreq; ++r)
+    if (!plans[r] || !V[r] || !psi[r] || S[r] <= 0 || nprobe[r] <= 0 || !out[r]) return EM_ERR_INVALID;
+  hipStream_t user = reinterpret_cast<hipStream_t>(hip_stream);
+  em_plan p0 = plans[0];
+  // size every buffer before capture
+  for (int r = 0; r < nreq; ++r) {
+    em_plan p = plans[r];
+    const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+    EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)nprobe[r] * ndet * ntiles));
+  }
+  hipEvent_t fork, join;
+  std::vector<hipEvent_t> done(nreq);
+  EM_HIP(p0, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  EM_HIP(p0, hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  for (auto& e : done) EM_HIP(p0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  EM_HIP(p0, hipEventRecord(fork, user));
+  EM_HIP(p0, hipStreamWaitEvent(p0->gstream, fork, 0));
+  Graph graph;
+  em_status st = p0->hip(hipStreamBeginCapture(p0->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
+  hipEvent_t cap_fork;
+  if (st == EM_OK) st = p0->hip(hipEventCreateWithFlags(&cap_fork, hipEventDisableTiming), "event");
+  if (st == EM_OK) st = p0->hip(hipEventRecord(cap_fork, p0->gstream), "record");
+  for (int r = 0; r < nreq && st == EM_OK; ++r) {
+    em_plan p = plans[r];
+    // each request is a branch of the graph on its plan's private stream
+    hipStream_t br = (r == 0) ? p0->gstream : p->gstream;
+    if (r > 0 && br == p0->gstream) {
+      st = p0->fail(EM_ERR_INVALID, "requests must use distinct plans");
+      break;
+    }
+    if (r > 0) st = p->hip(hipStreamWaitEvent(br, cap_fork, 0), "fork");
+    if (st != EM_OK) break;
+    StreamSwap sw(p, br);
+    const long long N = p->N();
+    const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+    double dg2min[kMaxDet], dg2max[kMaxDet];
+    for (int d = 0; d < ndet; ++d) {
+      dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+      dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
+    }
+    float2* wv = reinterpret_cast<float2*>(psi[r]);
+    const int B = nprobe[r];
+    int io = 0;
+    for (int s = 0; s < S[r] && st == EM_OK; ++s) {
+      RowArgs ra = row_args(p, wv, wv, B);
+      ra.V = V[r] + (long long)s * N;
+      st = row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, ra);
+      if (st != EM_OK) break;
+      const bool rec = ((s + 1) % stride == 0) || s == S[r] - 1;
+      if (rec) {
+        ColArgs cd = col_args(p, wv, nullptr, B);
+        cd.ndet = ndet;
+        for (int d = 0; d < ndet; ++d) {
+          cd.dg2min[d] = dg2min[d];
+          cd.dg2max[d] = dg2max[d];
+        }
+        cd.dpart = p->dpart.as<float>();
+        st = col(p, COL_PRE_FFT | COL_DET, cd);
+        if (st == EM_OK)
+          st = p->hip(launch_det_reduce(B, ndet, ntiles, p->dpart.as<float>(), 1.f, 0,
+                                        out[r] + (long long)io * ndet * B, B, p->stream),
+                      "detector reduce");
+        ++io;
+      }
+      if (st == EM_OK && s < S[r] - 1)
+        st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, wv, wv, B));
+    }
+    if (st == EM_OK) st = p->hip(hipEventRecord(done[r], br), "record");
+    if (st == EM_OK && r > 0) st = p->hip(hipStreamWaitEvent(p0->gstream, done[r], 0), "join");
+  }
+  const hipError_t ec = hipStreamEndCapture(p0->gstream, &graph.g);
+  if (st == EM_OK) st = p0->hip(ec, "end capture");
+  if (st == EM_OK) st = p0->hip(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0), "instantiate");
+  if (st == EM_OK) st = p0->hip(hipGraphLaunch(graph.x, p0->gstream), "graph launch");
+  if (st == EM_OK) st = p0->hip(hipEventRecord(join, p0->gstream), "record");
+  if (st == EM_OK) st = p0->hip(hipStreamWaitEvent(user, join, 0), "join");
+  if (st == EM_OK) st = p0->hip(hipStreamSynchronize(p0->gstream), "sync");
+  (void)hipEventDestroy(fork);
+  (void)hipEventDestroy(join);
+  (void)hipEventDestroy(cap_fork);
+  for (auto& e : done) (void)hipEventDestroy(e);
+  return st;
+}
+
+}  // extern "C"
+
+extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int nprobe, int iters, float* row_ms,
+                                    float* col_ms) {
+  if (!p || !V || !wave || nprobe <= 0 || iters <= 0 || !row_ms || !col_ms) return EM_ERR_INVALID;
+  float2* w = reinterpret_cast<float2*>(wave);
+  hipEvent_t e[3];
+  for (auto& x : e) EM_HIP(p, hipEventCreate(&x));
+  RowArgs r = row_args(p, w, w, nprobe);
+  r.V = V;
+  const ColArgs c = col_args(p, w, w, nprobe);
+  em_status st = EM_OK;
+  // warm up both kernels once
+  st = row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r);
+  if (st == EM_OK) st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
+  if (st == EM_OK) st = p->hip(hipEventRecord(e[0], p->stream), "event");
+  for (int i = 0; i < iters && st == EM_OK; ++i) st = row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r);
+  if (st == EM_OK) st = p->hip(hipEventRecord(e[1], p->stream), "event");
+  for (int i = 0; i < iters && st == EM_OK; ++i)
+    st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
+  if (st == EM_OK) st = p->hip(hipEventRecord(e[2], p->stream), "event");
+  if (st == EM_OK) st = p->hip(hipEventSynchronize(e[2]), "event sync");
+  float a = 0, b = 0;
+  if (st == EM_OK) st = p->hip(hipEventElapsedTime(&a, e[0], e[1]), "elapsed");
+  if (st == EM_OK) st = p->hip(hipEventElapsedTime(&b, e[1], e[2]), "elapsed");
+  for (auto& x : e) (void)hipEventDestroy(x);
+  *row_ms = a / iters;
+  *col_ms = b / iters;
+  return st;
+}
+
+extern "C" em_status em_host_slicing(long long n, const double* z, double dz, int border, int max_slices,
+                                     int* n_slices, double* bounds, long long* i0, long long* i1) {
+  if (n < 0 || (n > 0 && !z) || !(dz > 0) || border < 0 || border > 2 || !n_slices) return EM_ERR_INVALID;
+  for (long long i = 1; i < n; ++i)
+    if (z[i] < z[i - 1]) return EM_ERR_INVALID;
+  const std::vector<double> zz(z, z + n);
+  const std::vector<SliceRange> s = make_slices(zz, dz, border, kZEps);
+  *n_slices = (int)s.size();
+  if ((int)s.size() > max_slices) return EM_ERR_INVALID;
+  for (size_t k = 0; k < s.size(); ++k) {
+    if (bounds) {
+      bounds[k] = s[k].z_start;
+      bounds[k + 1] = s[k].z_end;
+    }
+    if (i0) i0[k] = s[k].i0;
+    if (i1) i1[k] = s[k].i1;
+  }
+  return EM_OK;
+}
+
+extern "C" em_status em_host_spline(int Z, int nR, double r_min, double r_max, double tap, double* R2, double* c0,
+                                    double* c1, double* c2, double* c3) {
+  if (Z <= 0 || Z > 103 || nR < 4 || nR > 256 || !(r_min > 0) || !(r_max > r_min) || !(tap > 0) || !(tap < 1))
+    return EM_ERR_INVALID;
+  const Spline s = build_spline(moliere_coefs(Z), nR, r_min, r_max, tap);
+  for (int i = 0; i < nR; ++i) {
+    if (R2) R2[i] = s.R2[i];
+    if (c0) c0[i] = s.c0[i];
+    if (c1) c1[i] = s.c1[i];
+    if (c2) c2[i] = s.c2[i];
+    if (c3) c3[i] = s.c3[i];
+  }
+  return EM_OK;
+}
+
+extern "C" em_status em_host_constants(double e0_kv, double* lambda, double* gamma, double* sigma) {
+  if (!(e0_kv > 0)) return EM_ERR_INVALID;
+  const Constants k = beam_constants(e0_kv);
+  if (lambda) *lambda = k.lambda;
+  if (gamma) *gamma = k.gamma;
+  if (sigma) *sigma = k.sigma;
+  return EM_OK;
+}

@@ -1,0 +1,303 @@
+// em_fft.hpp -- in-register / LDS Stockham FFT engine for gfx950.
+//
+// One FFT line of length N is owned by T threads; each thread holds E = N/T
+// complex values in registers in the "strided" distribution: thread tau holds
+// element tau + T*e in v[e].  A line is transformed by a sequence of Stockham
+// radix-R stages (R divides E, product of the radices = N).  Stage inputs are
+// read in the strided distribution and stage outputs are scattered to their
+// Stockham positions; the first stage reads straight from registers and the
+// last stage leaves its outputs in registers in the strided distribution
+// again, so a line makes (stages - 1) round trips through LDS and none through
+// HBM.  Because input and output share one distribution, an inverse FFT, a
+// pointwise multiply and a forward FFT chain in registers with no reordering.
+//
+// Directions: DIR = -1 forward exp(-2 pi i nk/N), DIR = +1 inverse, both
+// unnormalised, matching the C2C convention the reference relies on
+// (fft_dev.cuh:233-264).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace em {
+
+// --------------------------------------------------------------------------
+// complex helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float cnorm(float2 a) { return a.x * a.x + a.y * a.y; }
+// multiply by DIR*i
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+// multiply by exp(DIR * 2 pi i m / n) for a twiddle from the forward table
+template <int DIR>
+__device__ __forceinline__ float2 twmul(float2 a, float2 w_fwd) {
+  return DIR < 0 ? cmul(a, w_fwd) : cmulc(a, w_fwd);
+}
+
+// --------------------------------------------------------------------------
+// compile-time twiddle constants exp(DIR 2 pi i M / N)
+// --------------------------------------------------------------------------
+constexpr double kPiD = 3.141592653589793238462643383279502884;
+constexpr double cx_sin(double x) {
+  double term = x, sum = x;
+  for (int i = 1; i < 24; ++i) {
+    term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int i = 1; i < 24; ++i) {
+    term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+    sum += term;
+  }
+  return sum;
+}
+template <int N, int M, int DIR>
+struct CTw {
+  static constexpr int m = ((M % N) + N) % N;
+  // reduce the angle to [-pi, pi]
+  static constexpr double ang = DIR * 2.0 * kPiD * (2 * m <= N ? (double)m / N : (double)(m - N) / N);
+  static constexpr float re = (float)cx_cos(ang);
+  static constexpr float im = (float)cx_sin(ang);
+};
+
+template <int N, int M, int DIR>
+__device__ __forceinline__ float2 ctw(float2 a) {
+  constexpr int m = ((M % N) + N) % N;
+  if constexpr (m == 0) {
+    return a;
+  } else if constexpr (4 * m == N) {  // exp(DIR i pi/2) = DIR*i
+    return mul_i<DIR>(a);
+  } else if constexpr (2 * m == N) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (4 * m == 3 * N) {
+    return mul_i<-DIR>(a);
+  } else {
+    constexpr float c = CTw<N, M, DIR>::re, s = CTw<N, M, DIR>::im;
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+// --------------------------------------------------------------------------
+// small DFTs on compile-time register slots v[B + n*S], natural order in/out
+// --------------------------------------------------------------------------
+template <int R, int DIR, int B, int S, int E>
+struct Dft;
+
+template <int DIR, int B, int S, int E>
+struct Dft<2, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) {
+    const float2 a = v[B], b = v[B + S];
+    v[B] = cadd(a, b);
+    v[B + S] = csub(a, b);
+  }
+};
+
+template <int DIR, int B, int S, int E>
+struct Dft<3, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) {
+    constexpr float h = 0.86602540378443864676f;  // sqrt(3)/2
+    const float2 a0 = v[B], a1 = v[B + S], a2 = v[B + 2 * S];
+    const float2 s = cadd(a1, a2), d = csub(a1, a2);
+    const float2 m = make_float2(a0.x - 0.5f * s.x, a0.y - 0.5f * s.y);
+    const float2 r = cscale(mul_i<DIR>(d), h);
+    v[B] = cadd(a0, s);
+    v[B + S] = cadd(m, r);
+    v[B + 2 * S] = csub(m, r);
+  }
+};
+
+template <int DIR, int B, int S, int E>
+struct Dft<4, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) {
+    const float2 a0 = v[B], a1 = v[B + S], a2 = v[B + 2 * S], a3 = v[B + 3 * S];
+    const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    const float2 t2 = cadd(a1, a3), t3 = mul_i<DIR>(csub(a1, a3));
+    v[B] = cadd(t0, t2);
+    v[B + 2 * S] = csub(t0, t2);
+    v[B + S] = cadd(t1, t3);
+    v[B + 3 * S] = csub(t1, t3);
+  }
+};
+
+// Composite R = R1*R2 by the four-step split n = R2*n1 + n2, k = k1 + R1*k2.
+template <int R1, int R2, int DIR, int B, int S, int E>
+struct DftComposite {
+  static constexpr int R = R1 * R2;
+  template <int N2>
+  __device__ __forceinline__ static void step1(float2 (&v)[E]) {
+    if constexpr (N2 < R2) {
+      Dft<R1, DIR, B + N2 * S, R2 * S, E>::run(v);
+      step1<N2 + 1>(v);
+    }
+  }
+  template <int N2, int K1>
+  __device__ __forceinline__ static void step2(float2 (&v)[E]) {
+    if constexpr (N2 < R2) {
+      if constexpr (K1 < R1) {
+        constexpr int idx = B + N2 * S + K1 * R2 * S;
+        v[idx] = ctw<R, N2 * K1, DIR>(v[idx]);
+        step2<N2, K1 + 1>(v);
+      } else {
+        step2<N2 + 1, 0>(v);
+      }
+    }
+  }
+  template <int K1>
+  __device__ __forceinline__ static void step3(float2 (&v)[E]) {
+    if constexpr (K1 < R1) {
+      Dft<R2, DIR, B + K1 * R2 * S, S, E>::run(v);
+      step3<K1 + 1>(v);
+    }
+  }
+  __device__ __forceinline__ static void run(float2 (&v)[E]) {
+    step1<0>(v);
+    step2<0, 0>(v);
+    step3<0>(v);
+    // X[k1 + R1 k2] sits at slot R2 k1 + k2: reorder to natural order
+    float2 tmp[R];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) tmp[k1 + R1 * k2] = v[B + (R2 * k1 + k2) * S];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[B + k * S] = tmp[k];
+  }
+};
+
+template <int DIR, int B, int S, int E>
+struct Dft<8, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<2, 4, DIR, B, S, E>::run(v); }
+};
+template <int DIR, int B, int S, int E>
+struct Dft<16, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<4, 4, DIR, B, S, E>::run(v); }
+};
+template <int DIR, int B, int S, int E>
+struct Dft<6, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<2, 3, DIR, B, S, E>::run(v); }
+};
+template <int DIR, int B, int S, int E>
+struct Dft<12, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<3, 4, DIR, B, S, E>::run(v); }
+};
+template <int DIR, int B, int S, int E>
+struct Dft<32, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<4, 8, DIR, B, S, E>::run(v); }
+};
+
+// --------------------------------------------------------------------------
+// LDS layout of one line: one pad slot every 16 elements keeps the scattered
+// Stockham writes of radix-16 stages off a single bank.
+// --------------------------------------------------------------------------
+template <int N>
+struct LineLds {
+  static constexpr int size = N + N / 16;
+  __device__ __forceinline__ static int idx(int i) { return i + (i >> 4); }
+};
+
+// Synchronisation between the LDS write and read phases of an exchange.
+// SYNC_BLOCK: lines span several wavefronts (workgroup barrier).
+// SYNC_WAVE:  every line lives inside one wavefront; LDS requests of one wave
+//             complete in order, so only compiler reordering must be fenced.
+enum { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+template <int SYNC>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (SYNC == SYNC_BLOCK) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// --------------------------------------------------------------------------
+// Stockham stages.  Stage (R, Ns): butterfly j in [0, N/R), thread tau owns
+// j = tau + T*u, u in [0, E/R); its inputs x[j + r N/R] sit in v[u + r E/R];
+// twiddle W_{Ns R}^{r (j mod Ns)}; outputs go to (j/Ns) Ns R + (j mod Ns) + r Ns.
+// Twiddles come from tw[m] = exp(-2 pi i m / N) (FP64-generated table).
+// --------------------------------------------------------------------------
+template <int N, int T, int DIR, int SYNC>
+struct FftLine {
+  static constexpr int E = N / T;
+
+  template <int R, int U>
+  __device__ __forceinline__ static void butterflies(float2 (&v)[E]) {
+    if constexpr (U < E / R) {
+      Dft<R, DIR, U, E / R, E>::run(v);
+      butterflies<R, U + 1>(v);
+    }
+  }
+
+  template <int R, int NS>
+  __device__ __forceinline__ static void twiddle(float2 (&v)[E], int tau, const float2* __restrict__ tw) {
+    if constexpr (NS > 1) {
+      constexpr int step = N / (NS * R);
+#pragma unroll
+      for (int u = 0; u < E / R; ++u) {
+        const int j = tau + T * u;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float2 w = tw[(r * k * step) % N];
+          v[u + r * (E / R)] = twmul<DIR>(v[u + r * (E / R)], w);
+        }
+      }
+    }
+  }
+
+  template <int R, int NS>
+  __device__ __forceinline__ static void store_stage(const float2 (&v)[E], float2* lds, int tau) {
+#pragma unroll
+    for (int u = 0; u < E / R; ++u) {
+      const int j = tau + T * u;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[LineLds<N>::idx(base + r * NS)] = v[u + r * (E / R)];
+    }
+  }
+
+  __device__ __forceinline__ static void load_strided(float2 (&v)[E], const float2* lds, int tau) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lds[LineLds<N>::idx(tau + T * e)];
+  }
+
+  template <int NS, int R, int... RS>
+  __device__ __forceinline__ static void stages(float2 (&v)[E], float2* lds, int tau,
+                                                const float2* __restrict__ tw) {
+    static_assert(E % R == 0, "radix must divide the elements per thread");
+    twiddle<R, NS>(v, tau, tw);
+    butterflies<R, 0>(v);
+    if constexpr (sizeof...(RS) > 0) {
+      store_stage<R, NS>(v, lds, tau);
+      lds_sync<SYNC>();
+      load_strided(v, lds, tau);
+      lds_sync<SYNC>();
+      stages<NS * R, RS...>(v, lds, tau, tw);
+    } else {
+      static_assert(NS * R == N, "radices must multiply to N");
+    }
+  }
+
+  template <int... RS>
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2* lds, int tau,
+                                             const float2* __restrict__ tw) {
+    stages<1, RS...>(v, lds, tau, tw);
+  }
+};
+
+}  // namespace em

@@ -1,0 +1,44 @@
+// em_host.hpp -- host-side physics of the engine: constants, element tables,
+// spline builder, slicing, lens derivation.  Double precision throughout; the
+// device only sees the FP32 / FP64 results.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "cudaem_amd.h"
+#include "em_kernels.hpp"
+
+namespace em {
+
+struct Constants {
+  double lambda, gamma, sigma;
+};
+Constants beam_constants(double e0_kv);
+
+// Kirkland-form projected-potential coefficients of one element.
+struct ElementCoefs {
+  std::vector<double> a, b, c, d;
+};
+ElementCoefs moliere_coefs(int Z);
+void projected_potential(const ElementCoefs& k, double R, double* V, double* dVdR);
+
+struct Spline {
+  std::vector<double> R2, c0, c1, c2, c3;
+  double rmax2;
+};
+Spline build_spline(const ElementCoefs& k, int nR, double r_min, double r_max, double tap_frac);
+
+struct SliceRange {
+  double z_start, z_end;
+  long long i0, i1;  // inclusive atom range; i0 > i1 when empty
+};
+// z sorted ascending.
+std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int border, double eps);
+
+LensDev derive_lens(const em_lens& l, double lambda, double defocus_shift);
+// aperture / detector semi-angle (mrad) -> g^2 (1/A^2); outer <= 0 -> +inf
+double mrad_to_g2(double mrad, double lambda, bool outer);
+
+GridDev make_grid_dev(const em_grid& g);
+
+}  // namespace em

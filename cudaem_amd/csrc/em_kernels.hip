@@ -1,0 +1,734 @@
+// em_kernels.hip -- gfx950 kernels of the multislice hot path.
+//
+// The slice step psi <- IFFT2[P . FFT2[t . psi]] (wave_function.cuh:60-87,
+// fftcallbacks.cuh:11-64) is split into two fused passes over the wave:
+//   row pass    (per row y):    IFFT_x -> multiply t_k(y, x) -> FFT_x
+//   column pass (per column kx): FFT_y -> multiply P(ky, kx) -> IFFT_y
+// so one slice is one row pass plus one column pass, each reading and writing
+// the wave once.  Between passes the wave is kept in (y, kx) layout, row-major.
+// Both passes reuse the transmission / propagator values held in registers
+// for every probe a workgroup processes.
+#include "em_fft.hpp"
+#include "em_kernels.hpp"
+
+namespace em {
+
+// --------------------------------------------------------------------------
+// FFT configurations per line length: threads per line and radix sequence.
+// --------------------------------------------------------------------------
+template <int N>
+struct FftCfg;
+#define EM_FFT_CFG(N_, T_, ...)                                                     \
+  template <>                                                                       \
+  struct FftCfg<N_> {                                                               \
+    static constexpr int T = T_;                                                    \
+    template <int DIR, int SYNC>                                                    \
+    __device__ __forceinline__ static void run(float2 (&v)[N_ / T_], float2* lds,   \
+                                               int tau, const float2* tw) {         \
+      FftLine<N_, T_, DIR, SYNC>::template run<__VA_ARGS__>(v, lds, tau, tw);       \
+    }                                                                               \
+  };
+EM_FFT_CFG(64, 8, 8, 8)
+EM_FFT_CFG(128, 16, 8, 8, 2)
+EM_FFT_CFG(256, 16, 16, 16)
+EM_FFT_CFG(512, 64, 8, 8, 8)
+EM_FFT_CFG(768, 64, 12, 4, 4, 4)
+EM_FFT_CFG(1024, 64, 16, 16, 4)
+EM_FFT_CFG(2048, 128, 16, 16, 8)
+#undef EM_FFT_CFG
+
+constexpr int kRowThreads = 256;
+
+// Columns per column-pass workgroup for a column length N.
+template <int N>
+struct ColCfg {
+  static constexpr int T = FftCfg<N>::T;
+  static constexpr int COLS = (T >= 128) ? 8 : 16;
+  static constexpr int THREADS = COLS * T;
+  static constexpr int STRIDE = LineLds<N>::size + 1;  // odd stride: columns on distinct banks
+  static constexpr size_t LDS = (size_t)COLS * STRIDE * sizeof(float2);
+};
+
+// --------------------------------------------------------------------------
+// Row pass
+// --------------------------------------------------------------------------
+template <int NX, unsigned MODE>
+__global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
+  constexpr int T = FftCfg<NX>::T;
+  constexpr int E = NX / T;
+  constexpr int LINES = kRowThreads / T;
+  extern __shared__ float2 smem[];
+  const int line = threadIdx.x / T;
+  const int tau = threadIdx.x % T;
+  const int y = blockIdx.x * LINES + line;
+  const int b0 = blockIdx.y * a.pb;
+  const int b1 = min(a.nprobe, b0 + a.pb);
+  float2* lds = smem + line * LineLds<NX>::size;
+  const long long row = (long long)y * NX;
+
+  // transmission values of this thread's elements, reused for every probe
+  float2 t[E];
+  if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const long long i = row + tau + T * e;
+      if constexpr ((MODE & ROW_MUL_V) != 0) {
+        float s, c;
+        sincosf(a.V[i], &s, &c);
+        t[e] = make_float2(c, s);
+      } else {
+        t[e] = a.t[i];
+      }
+      if constexpr ((MODE & ROW_CONJ) != 0) t[e].y = -t[e].y;
+    }
+  }
+  float2 g[(MODE & ROW_GRAD) != 0 ? E : 1];
+  if constexpr ((MODE & ROW_GRAD) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) g[e] = make_float2(0.f, 0.f);
+  }
+
+  for (int b = b0; b < b1; ++b) {
+    const long long base = (long long)b * a.pstride + row;
+    float2 v[E];
+    if constexpr ((MODE & ROW_IN_PLANE) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = a.in[base + tau + T * e];
+    }
+    if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.ckpt[base + tau + T * e] = v[e];
+    }
+    if constexpr ((MODE & ROW_GRAD) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 p = a.ckpt[base + tau + T * e];
+        // conj(p) * v
+        g[e].x += p.x * v[e].x + p.y * v[e].y;
+        g[e].y += p.x * v[e].y - p.y * v[e].x;
+      }
+    }
+    if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = cmul(v[e], t[e]);
+    }
+    if constexpr ((MODE & ROW_POST_FFT) != 0) FftCfg<NX>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & ROW_OUT_ABS2) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long i = base + tau + T * e;
+        const float val = cnorm(v[e]);
+        if constexpr ((MODE & ROW_OUT_ACCUM) != 0) a.o1[i] += a.wgt * val; else a.o1[i] = a.wgt * val;
+      }
+    } else if constexpr ((MODE & ROW_OUT_SPLIT) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long i = base + tau + T * e;
+        if constexpr ((MODE & ROW_OUT_ACCUM) != 0) {
+          a.o1[i] += a.wgt * v[e].x;
+          a.o2[i] += a.wgt * v[e].y;
+        } else {
+          a.o1[i] = a.wgt * v[e].x;
+          a.o2[i] = a.wgt * v[e].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.out[base + tau + T * e] = v[e];
+    }
+  }
+  if constexpr ((MODE & ROW_GRAD) != 0) {
+    float2* gp = a.gpart + (long long)blockIdx.y * a.ny * NX + row;
+#pragma unroll
+    for (int e = 0; e < E; ++e) gp[tau + T * e] = g[e];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Column pass
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
+  const int sx = ix < g.nx / 2 ? ix : ix - g.nx;
+  const int sy = iy < g.ny / 2 ? iy : iy - g.ny;
+  const double gx = __dmul_rn((double)sx, g.dgx);
+  const double gy = __dmul_rn((double)sy, g.dgy);
+  return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+}
+
+template <int NY, unsigned MODE>
+__global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a) {
+  using C = ColCfg<NY>;
+  constexpr int T = C::T;
+  constexpr int E = NY / T;
+  constexpr int COLS = C::COLS;
+  extern __shared__ float2 smem[];
+  const int c = threadIdx.x % COLS;
+  const int tau = threadIdx.x / COLS;
+  const int x = blockIdx.x * COLS + c;
+  const int nx = a.nx;
+  const int b0 = blockIdx.y * a.pb;
+  const int b1 = min(a.nprobe, b0 + a.pb);
+  float2* lds = smem + c * C::STRIDE;
+
+  float2 p[(MODE & (COL_MUL_P | COL_CONJ_P | COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0 ? E : 1];
+  if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      p[e] = a.P[(long long)(tau + T * e) * nx + x];
+      if constexpr ((MODE & COL_CONJ_P) != 0) p[e].y = -p[e].y;
+    }
+  } else if constexpr ((MODE & (COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) p[e] = a.A[(long long)(tau + T * e) * nx + x];
+  }
+  unsigned fmask = 0;
+  if constexpr ((MODE & COL_MUL_FILTER) != 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double g2 = band_g2(x, tau + T * e, a.grid);
+      if (a.fg2min <= g2 && g2 <= a.fg2max) fmask |= 1u << e;
+    }
+  }
+  unsigned dmask[(MODE & COL_DET) != 0 ? kMaxDet : 1];
+  if constexpr ((MODE & COL_DET) != 0) {
+    for (int d = 0; d < kMaxDet; ++d) dmask[d] = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double g2 = band_g2(x, tau + T * e, a.grid);
+      for (int d = 0; d < a.ndet; ++d)
+        if (a.dg2min[d] <= g2 && g2 <= a.dg2max[d]) dmask[d] |= 1u << e;
+    }
+  }
+
+  for (int b = b0; b < b1; ++b) {
+    const long long base = (long long)b * a.pstride + x;
+    float2 v[E];
+    if constexpr ((MODE & COL_LOAD_PROBE) != 0) {
+      const float2 exv = a.ex[(long long)b * nx + x];
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 eyv = a.ey[(long long)b * NY + tau + T * e];
+        v[e] = cmul(cmul(p[e], exv), eyv);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = a.in[base + (long long)(tau + T * e) * nx];
+      if constexpr ((MODE & COL_LOAD_MUL_A) != 0) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
+      }
+    }
+    if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
+    }
+    if constexpr ((MODE & COL_MUL_FILTER) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        v[e] = ((fmask >> e) & 1u) ? cscale(v[e], a.fscale) : make_float2(0.f, 0.f);
+    }
+    if constexpr ((MODE & COL_DET) != 0) {
+      __shared__ float red[kMaxDet][C::THREADS / 64];
+      const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      for (int d = 0; d < a.ndet; ++d) {
+        float s = 0.f;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+          if ((dmask[d] >> e) & 1u) s += cnorm(v[e]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) red[d][wv] = s;
+      }
+      __syncthreads();
+      if (threadIdx.x < a.ndet) {
+        float s = 0.f;
+        for (int w = 0; w < C::THREADS / 64; ++w) s += red[threadIdx.x][w];
+        a.dpart[((long long)b * a.ndet + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+      }
+      __syncthreads();
+    }
+    if constexpr ((MODE & COL_POST_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & COL_STORE) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.out[base + (long long)(tau + T * e) * nx] = v[e];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Mode tables: every (size, mode) the host layer launches is instantiated here.
+// --------------------------------------------------------------------------
+#define EM_ROW_MODES(X)                                                         \
+  X(ROW_POST_FFT)                                                               \
+  X(ROW_PRE_IFFT)                                                               \
+  X(ROW_MUL_V | ROW_POST_FFT)                                                   \
+  X(ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT)                                    \
+  X(ROW_IN_PLANE | ROW_MUL_V | ROW_POST_FFT)                                    \
+  X(ROW_MUL_T | ROW_POST_FFT)                                                   \
+  X(ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT)                                    \
+  X(ROW_PRE_IFFT | ROW_CKPT_STORE | ROW_MUL_T | ROW_POST_FFT)                   \
+  X(ROW_CKPT_STORE | ROW_MUL_T | ROW_POST_FFT)                                  \
+  X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_T | ROW_CONJ | ROW_POST_FFT)              \
+  X(ROW_PRE_IFFT | ROW_GRAD)                                                    \
+  X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_T | ROW_CONJ)                             \
+  X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_V | ROW_CONJ)                             \
+  X(ROW_PRE_IFFT | ROW_CKPT_STORE | ROW_MUL_V | ROW_POST_FFT)                   \
+  X(ROW_CKPT_STORE | ROW_MUL_V | ROW_POST_FFT)                                  \
+  X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_V | ROW_CONJ | ROW_POST_FFT)              \
+  X(ROW_PRE_IFFT | ROW_OUT_ABS2)                                                \
+  X(ROW_PRE_IFFT | ROW_OUT_ABS2 | ROW_OUT_ACCUM)                                \
+  X(ROW_PRE_IFFT | ROW_OUT_SPLIT)                                               \
+  X(ROW_PRE_IFFT | ROW_OUT_SPLIT | ROW_OUT_ACCUM)
+
+#define EM_COL_MODES(X)                                                         \
+  X(COL_DET)                                                                    \
+  X(COL_PRE_FFT | COL_STORE)                                                    \
+  X(COL_PRE_IFFT | COL_STORE)                                                   \
+  X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
+  X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)                       \
+  X(COL_PRE_FFT | COL_DET)                                                      \
+  X(COL_PRE_FFT | COL_DET | COL_STORE)                                          \
+  X(COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE)                                 \
+  X(COL_LOAD_MUL_A | COL_POST_IFFT | COL_STORE)                                 \
+  X(COL_PRE_FFT | COL_MUL_FILTER | COL_POST_IFFT | COL_STORE)                   \
+  X(COL_PRE_IFFT | COL_CONJ_P | COL_POST_FFT | COL_STORE)                       \
+  X(COL_PRE_IFFT | COL_MUL_P | COL_POST_FFT | COL_STORE)
+
+#define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define EM_CASE(N) case N:
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int NX>
+static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
+  constexpr int LINES = kRowThreads / FftCfg<NX>::T;
+  const dim3 grid(a.ny / LINES, (a.nprobe + a.pb - 1) / a.pb);
+  const size_t lds = (size_t)LINES * LineLds<NX>::size * sizeof(float2);
+  switch (mode) {
+#define EM_ROW_CASE(M)                                                            \
+  case (M):                                                                       \
+    hipLaunchKernelGGL((row_pass_kernel<NX, (M)>), grid, dim3(kRowThreads), lds, s, a); \
+    return hipGetLastError();
+    EM_ROW_MODES(EM_ROW_CASE)
+#undef EM_ROW_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int NY, unsigned M>
+static hipError_t col_launch_one(const ColArgs& a, hipStream_t s) {
+  using C = ColCfg<NY>;
+  const dim3 grid(a.nx / C::COLS, (a.nprobe + a.pb - 1) / a.pb);
+  hipLaunchKernelGGL((col_pass_kernel<NY, M>), grid, dim3(C::THREADS), C::LDS, s, a);
+  return hipGetLastError();
+}
+
+template <int NY>
+static hipError_t col_attrs(void) {
+  hipError_t e = hipSuccess;
+#define EM_COL_ATTR(M)                                                                   \
+  if (e == hipSuccess)                                                                   \
+    e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, (M)>,                        \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ColCfg<NY>::LDS);
+  EM_COL_MODES(EM_COL_ATTR)
+#undef EM_COL_ATTR
+  return e;
+}
+
+hipError_t prepare_col_kernels(int ny) {
+  switch (ny) {
+#define EM_CASE(N) \
+  case N:          \
+    return col_attrs<N>();
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int NY>
+static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
+  switch (mode) {
+#define EM_COL_CASE(M) \
+  case (M):            \
+    return col_launch_one<NY, (M)>(a, s);
+    EM_COL_MODES(EM_COL_CASE)
+#undef EM_COL_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s) {
+  if (a.ny % row_lines_per_block(a.nx) != 0 || a.pb <= 0) return hipErrorInvalidValue;
+  switch (a.nx) {
+#define EM_CASE(N) \
+  case N:          \
+    return row_dispatch<N>(mode, a, s);
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s) {
+  if (a.nx % col_tile(a.ny) != 0 || a.pb <= 0 || a.ndet > kMaxDet) return hipErrorInvalidValue;
+  switch (a.ny) {
+#define EM_CASE(N) \
+  case N:          \
+    return col_dispatch<N>(mode, a, s);
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int row_lines_per_block(int nx) {
+  switch (nx) {
+#define EM_CASE(N) \
+  case N:          \
+    return kRowThreads / FftCfg<N>::T;
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return 1;
+  }
+}
+
+int col_tile(int ny) {
+  switch (ny) {
+#define EM_CASE(N) \
+  case N:          \
+    return ColCfg<N>::COLS;
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return 1;
+  }
+}
+
+int col_ntiles(int nx, int ny) { return nx / col_tile(ny); }
+
+// --------------------------------------------------------------------------
+// Projected potential of every stored slice, gen_slice_potential_kernel
+// (device_kernels.cuh:28-47) as a binned gather: a 16x16 pixel tile visits
+// only the atom bins within R_max of it.  Atoms: (x, y, occ, type bits),
+// sorted by (slice, bin); V[s] += occ * S_type(R^2) for R^2 < R_max^2 with the
+// non-periodic distance of grid_dev.cuh:135-154.  The sigma scaling is folded
+// into occ by the host (coeff*occ, device_kernels.cuh:41).  Spline tables
+// (5 x nR floats per type: R2, c0..c3) are staged in LDS.
+// --------------------------------------------------------------------------
+constexpr int kPotTile = 16;
+constexpr int kMaxTypes = 8;
+constexpr int kMaxNR = 256;
+
+__global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
+    GridDev g, const int* __restrict__ off, int nbx, int nby, float bin, const float4* __restrict__ atoms,
+    int ntypes, int nR, const float* __restrict__ tables, const float* __restrict__ rmax2, float* V) {
+  __shared__ float tab[kMaxTypes * 5 * kMaxNR];
+  __shared__ float srm2[kMaxTypes];
+  for (int i = threadIdx.x; i < ntypes * 5 * nR; i += blockDim.x) tab[i] = tables[i];
+  if (threadIdx.x < ntypes) srm2[threadIdx.x] = rmax2[threadIdx.x];
+  __syncthreads();
+  const int s = blockIdx.z;
+  const int ix = blockIdx.x * kPotTile + (threadIdx.x % kPotTile);
+  const int iy = blockIdx.y * kPotTile + (threadIdx.x / kPotTile);
+  const float px = ix * g.dx, py = iy * g.dy;
+  // bin range touched by this tile (bins start at -bin)
+  float rm = 0.f;
+  for (int t = 0; t < ntypes; ++t) rm = fmaxf(rm, srm2[t]);
+  rm = sqrtf(rm);
+  const float tx0 = blockIdx.x * kPotTile * g.dx, tx1 = (blockIdx.x * kPotTile + kPotTile - 1) * g.dx;
+  const float ty0 = blockIdx.y * kPotTile * g.dy, ty1 = (blockIdx.y * kPotTile + kPotTile - 1) * g.dy;
+  const int bx0 = max(0, (int)floorf((tx0 - rm + bin) / bin));
+  const int bx1 = min(nbx - 1, (int)floorf((tx1 + rm + bin) / bin));
+  const int by0 = max(0, (int)floorf((ty0 - rm + bin) / bin));
+  const int by1 = min(nby - 1, (int)floorf((ty1 + rm + bin) / bin));
+  const int* so = off + (long long)s * (nbx * nby + 1);
+  float acc = 0.f;
+  for (int by = by0; by <= by1; ++by) {
+    const int a0 = so[by * nbx + bx0], a1 = so[by * nbx + bx1 + 1];
+    for (int ia = a0; ia < a1; ++ia) {
+      const float4 at = atoms[ia];
+      const float ddx = px - at.x, ddy = py - at.y;
+      const float r2 = ddx * ddx + ddy * ddy;
+      const int t = __float_as_int(at.w);
+      if (r2 < srm2[t]) {
+        const float* R2 = tab + t * 5 * nR;
+        int i0 = 0, ie = nR - 1;
+        while (ie - i0 > 1) {
+          const int im = (i0 + ie) >> 1;
+          if (r2 < R2[im]) ie = im; else i0 = im;
+        }
+        const float dx = r2 - R2[i0];
+        const float* cf = R2 + nR;
+        const float val = ((cf[3 * nR + i0] * dx + cf[2 * nR + i0]) * dx + cf[nR + i0]) * dx + cf[i0];
+        acc += at.z * val;
+      }
+    }
+  }
+  if (ix < g.nx && iy < g.ny) V[(long long)s * g.nx * g.ny + (long long)iy * g.nx + ix] = acc;
+}
+
+hipError_t launch_potential(const GridDev& g, int nslices, const int* off, int nbx, int nby,
+                            float bin, const float4* atoms, int ntypes, int nR,
+                            const float* tables, const float* rmax2, float* V, hipStream_t s) {
+  if (ntypes > kMaxTypes || nR > kMaxNR || nslices <= 0) return hipErrorInvalidValue;
+  const dim3 grid((g.nx + kPotTile - 1) / kPotTile, (g.ny + kPotTile - 1) / kPotTile, nslices);
+  hipLaunchKernelGGL(potential_kernel, grid, dim3(kPotTile * kPotTile), 0, s, g, off, nbx, nby, bin,
+                     atoms, ntypes, nR, tables, rmax2, V);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Propagator P = (1/N) bwl exp(i w |g - g0|^2), w = -pi lambda dz
+// (gen_propagator_kernel, device_kernels.cuh:215-234; bwl_factor_shift,
+// grid_dev.cuh:193-196).  FP64 phase: at 1024^2 it reaches tens of radians.
+// --------------------------------------------------------------------------
+__global__ void propagator_kernel(GridDev g, double w, double gx0, double gy0, int bwl, float2* P) {
+  const long long n = (long long)g.nx * g.ny;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % g.nx), iy = (int)(i / g.nx);
+    const int sx = ix < g.nx / 2 ? ix : ix - g.nx;
+    const int sy = iy < g.ny / 2 ? iy : iy - g.ny;
+    const double gx = sx * g.dgx - gx0, gy = sy * g.dgy - gy0;
+    double amp = 1.0 / (double)n;
+    if (bwl) {
+      const double gu2 = (sx * g.dgx) * (sx * g.dgx) + (sy * g.dgy) * (sy * g.dgy);
+      amp *= 1.0 / (1.0 + exp(g.alpha * (gu2 - g.gl2_max)));
+    }
+    double sn, cs;
+    sincos(w * (gx * gx + gy * gy), &sn, &cs);
+    P[i] = make_float2((float)(amp * cs), (float)(amp * sn));
+  }
+}
+
+hipError_t launch_propagator(const GridDev& g, double w, double gx0, double gy0, int bwl, float2* P,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(propagator_kernel, dim3(1024), dim3(256), 0, s, g, w, gx0, gy0, bwl, P);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Aperture-limited lens phase table A(g) = [g2_min <= g^2 < g2_max] exp(i chi(g))
+// (Generate_probe / apply_CTF_kernel, device_kernels.cuh:363-388, 445-468;
+// coherent_phase_shift, basic_functions_dev.cuh:215-234), in FP64.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ bool lens_nz(double v) { return fabs(v) > 1e-5; }
+
+__device__ double lens_chi(const LensDev& l, double gx, double gy) {
+  const double g2 = gx * gx + gy * gy, g4 = g2 * g2, g6 = g4 * g2;
+  double ph = (lens_nz(l.cc10) ? l.cc10 * g2 : 0.0) + (lens_nz(l.cc30) ? l.cc30 * g4 : 0.0) +
+              (lens_nz(l.cc50) ? l.cc50 * g6 : 0.0);
+  if (l.phi_required) {
+    const double gg = sqrt(g2), g3 = g2 * gg, g5 = g4 * gg;
+    const double phi = atan2(gy, gx);
+    ph += lens_nz(phi) ? l.m * phi : 0.0;
+    ph += lens_nz(l.cc12) ? l.cc12 * g2 * sin(2 * (phi - l.phi12)) : 0.0;
+    ph += lens_nz(l.cc21) ? l.cc21 * g3 * sin(phi - l.phi21) : 0.0;
+    ph += lens_nz(l.cc23) ? l.cc23 * g3 * sin(3 * (phi - l.phi23)) : 0.0;
+    ph += lens_nz(l.cc32) ? l.cc32 * g4 * sin(2 * (phi - l.phi32)) : 0.0;
+    ph += lens_nz(l.cc34) ? l.cc34 * g4 * sin(4 * (phi - l.phi34)) : 0.0;
+    ph += lens_nz(l.cc41) ? l.cc41 * g5 * sin(phi - l.phi41) : 0.0;
+    ph += lens_nz(l.cc43) ? l.cc43 * g5 * sin(3 * (phi - l.phi43)) : 0.0;
+    ph += lens_nz(l.cc45) ? l.cc45 * g5 * sin(5 * (phi - l.phi45)) : 0.0;
+    ph += lens_nz(l.cc52) ? l.cc52 * g6 * sin(2 * (phi - l.phi52)) : 0.0;
+    ph += lens_nz(l.cc54) ? l.cc54 * g6 * sin(4 * (phi - l.phi54)) : 0.0;
+    ph += lens_nz(l.cc56) ? l.cc56 * g6 * sin(6 * (phi - l.phi56)) : 0.0;
+  }
+  return ph;
+}
+
+__global__ void lens_table_kernel(GridDev g, LensDev l, float2* A) {
+  const long long n = (long long)g.nx * g.ny;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % g.nx), iy = (int)(i / g.nx);
+    const int sx = ix < g.nx / 2 ? ix : ix - g.nx;
+    const int sy = iy < g.ny / 2 ? iy : iy - g.ny;
+    const double gx = __dmul_rn((double)sx, g.dgx), gy = __dmul_rn((double)sy, g.dgy);
+    const double g2 = __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
+    float2 v = make_float2(0.f, 0.f);
+    if (l.g2_min <= g2 && g2 < l.g2_max) {
+      double sn, cs;
+      sincos(lens_chi(l, gx, gy), &sn, &cs);
+      v = make_float2((float)cs, (float)sn);
+    }
+    A[i] = v;
+  }
+}
+
+hipError_t launch_lens_table(const GridDev& g, const LensDev& l, float2* A, hipStream_t s) {
+  hipLaunchKernelGGL(lens_table_kernel, dim3(1024), dim3(256), 0, s, g, l, A);
+  return hipGetLastError();
+}
+
+// Per-probe position ramps exp(i xe gx), exp(i ye gy) with
+// xe = -2 pi (x_b - x_l/2) (grid_dev.cuh:104-111), FP64 phase.
+__global__ void probe_ramps_kernel(GridDev g, int nprobe, const double* xe, const double* ye,
+                                   float2* ex, float2* ey) {
+  const int b = blockIdx.y;
+  if (b >= nprobe) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nx + g.ny; i += gridDim.x * blockDim.x) {
+    double sn, cs;
+    if (i < g.nx) {
+      const int sx = i < g.nx / 2 ? i : i - g.nx;
+      sincos(xe[b] * (sx * g.dgx), &sn, &cs);
+      ex[(long long)b * g.nx + i] = make_float2((float)cs, (float)sn);
+    } else {
+      const int iy = i - g.nx;
+      const int sy = iy < g.ny / 2 ? iy : iy - g.ny;
+      sincos(ye[b] * (sy * g.dgy), &sn, &cs);
+      ey[(long long)b * g.ny + iy] = make_float2((float)cs, (float)sn);
+    }
+  }
+}
+
+hipError_t launch_probe_ramps(const GridDev& g, int nprobe, const double* xe, const double* ye,
+                              float2* ex, float2* ey, hipStream_t s) {
+  hipLaunchKernelGGL(probe_ramps_kernel, dim3(8, nprobe), dim3(256), 0, s, g, nprobe, xe, ye, ex, ey);
+  return hipGetLastError();
+}
+
+// out[d * ostride + b] (+)= w * sum_tiles part[b][d][tile]
+__global__ void det_reduce_kernel(int nprobe, int ndet, int ntiles, const float* part, float w,
+                                  int accumulate, float* out, long long ostride) {
+  const int i = blockIdx.x * blockDim.y + threadIdx.y;  // (b, d) pair
+  if (i >= nprobe * ndet) return;
+  const int b = i / ndet, d = i % ndet;
+  float s = 0.f;
+  for (int t = threadIdx.x; t < ntiles; t += 64) s += part[(long long)i * ntiles + t];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) {
+    float* o = out + (long long)d * ostride + b;
+    *o = accumulate ? (*o + w * s) : w * s;
+  }
+}
+
+hipError_t launch_det_reduce(int nprobe, int ndet, int ntiles, const float* part, float w,
+                             int accumulate, float* out, long long ostride, hipStream_t s) {
+  const int pairs = nprobe * ndet;
+  hipLaunchKernelGGL(det_reduce_kernel, dim3((pairs + 3) / 4), dim3(64, 4), 0, s, nprobe, ndet,
+                     ntiles, part, w, accumulate, out, ostride);
+  return hipGetLastError();
+}
+
+__global__ void grad_reduce_kernel(long long n, int nparts, const float2* part, float2* g) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float2 s = make_float2(0.f, 0.f);
+    for (int p = 0; p < nparts; ++p) s = cadd(s, part[(long long)p * n + i]);
+    g[i] = s;
+  }
+}
+
+hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g, hipStream_t s) {
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(2048), dim3(256), 0, s, n, nparts, part, g);
+  return hipGetLastError();
+}
+
+__global__ void fill_kernel(float2* p, long long n, float2 v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill(float2* p, long long n, float2 v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(2048), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+// Quadrant swap of even-sized planes (fft_shift, device_kernels.cuh:342-362),
+// out of place so in and out may differ.
+__global__ void fft_shift_kernel(const float* in, float* out, int nx, int ny, long long nb, float w,
+                                 int acc) {
+  const long long n = (long long)nx * ny;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n * nb;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long b = i / n, r = i % n;
+    const int ix = (int)(r % nx), iy = (int)(r / nx);
+    const int jx = (ix + nx / 2) % nx, jy = (iy + ny / 2) % ny;
+    float* o = out + b * n + (long long)jy * nx + jx;
+    *o = acc ? *o + w * in[i] : w * in[i];
+  }
+}
+
+hipError_t launch_fft_shift(const float* in, float* out, int nx, int ny, long long nb, float w, int acc,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(fft_shift_kernel, dim3(2048), dim3(256), 0, s, in, out, nx, ny, nb, w, acc);
+  return hipGetLastError();
+}
+
+__global__ void abs2_kernel(const float2* in, float* out, long long n, float w, int acc) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float v = cnorm(in[i]);
+    out[i] = acc ? out[i] + w * v : v;
+  }
+}
+
+hipError_t launch_abs2(const float2* in, float* out, long long n, float w, int acc, hipStream_t s) {
+  hipLaunchKernelGGL(abs2_kernel, dim3(2048), dim3(256), 0, s, in, out, n, w, acc);
+  return hipGetLastError();
+}
+
+__global__ void split_kernel(const float2* in, float* re, float* im, long long n, float w, int acc) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 v = in[i];
+    if (acc) {
+      re[i] += w * v.x;
+      im[i] += w * v.y;
+    } else {
+      re[i] = v.x;
+      im[i] = v.y;
+    }
+  }
+}
+
+hipError_t launch_split(const float2* in, float* re, float* im, long long n, float w, int acc,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(split_kernel, dim3(2048), dim3(256), 0, s, in, re, im, n, w, acc);
+  return hipGetLastError();
+}
+
+// dL/dV for t = exp(i V): Re(conj(gt) * i t) * sigma
+__global__ void grad_v_kernel(const float2* gt, const float* V, float sigma, long long n, float* gV) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s, c;
+    sincosf(V[i], &s, &c);
+    const float2 g = gt[i];
+    // i t = (-s, c); Re(conj(g) * (-s, c)) = -g.x s + g.y c
+    gV[i] = sigma * (-g.x * s + g.y * c);
+  }
+}
+
+hipError_t launch_grad_v(const float2* gt, const float* V, float sigma, long long n, float* gV,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(grad_v_kernel, dim3(2048), dim3(256), 0, s, gt, V, sigma, n, gV);
+  return hipGetLastError();
+}
+
+}  // namespace em

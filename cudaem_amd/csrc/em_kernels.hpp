@@ -1,0 +1,126 @@
+// em_kernels.hpp -- host-side declarations of the gfx950 kernels' launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace em {
+
+// Grid description passed by value to the kernels.
+struct GridDev {
+  int nx, ny;
+  float dx, dy;      // real-space sampling (Angstrom)
+  double dgx, dgy;   // reciprocal sampling (1/Angstrom), kept in FP64 so band
+                     // membership of a pixel is decided exactly as on the host
+  double gl2_max, alpha;  // band-width-limit Fermi factor (grid_dev.cuh:53-55)
+};
+
+// Lens in derived form (lens_dev.cuh:40-57), FP64: the probe and CTF tables
+// are built once per run, so the aberration phase is evaluated in double.
+struct LensDev {
+  double m, cc10, cc12, cc21, cc23, cc30, cc32, cc34, cc41, cc43, cc45, cc50, cc52, cc54, cc56;
+  double phi12, phi21, phi23, phi32, phi34, phi41, phi43, phi45, phi52, phi54, phi56;
+  double g2_min, g2_max;
+  int phi_required;
+};
+
+// --- row pass ---------------------------------------------------------------
+enum RowMode : unsigned {
+  ROW_PRE_IFFT = 1u << 0,    // inverse FFT along x before the multiply
+  ROW_POST_FFT = 1u << 1,    // forward FFT along x after the multiply
+  ROW_MUL_V = 1u << 2,       // multiply by exp(i V), V sigma-scaled FP32
+  ROW_MUL_T = 1u << 3,       // multiply by a complex transmission t
+  ROW_CONJ = 1u << 4,        // use conj of the transmission (adjoint)
+  ROW_IN_PLANE = 1u << 5,    // input is the plane wave 1 (no load)
+  ROW_CKPT_STORE = 1u << 6,  // store the real-space wave before the multiply
+  ROW_GRAD = 1u << 7,        // accumulate conj(ckpt) * wave before the multiply
+  ROW_OUT_ABS2 = 1u << 8,    // store |.|^2 (FP32) instead of the complex wave
+  ROW_OUT_SPLIT = 1u << 9,   // store re / im planes
+  ROW_OUT_ACCUM = 1u << 10,  // accumulate weight * output into the FP32 planes
+};
+
+struct RowArgs {
+  const float2* in;
+  float2* out;
+  long long pstride;        // elements between probes
+  int nx, ny, nprobe, pb;   // pb: probes per workgroup
+  const float* V;           // [ny][nx]
+  const float2* t;          // [ny][nx]
+  const float2* tw;         // [nx] exp(-2 pi i m / nx)
+  float2* ckpt;             // [nprobe][ny][nx]
+  float2* gpart;            // [gridDim.y][ny][nx] gradient partials
+  float* o1;                // [nprobe][ny][nx] intensity or real part
+  float* o2;                // [nprobe][ny][nx] imaginary part
+  float wgt;
+};
+
+// --- column pass ------------------------------------------------------------
+enum ColMode : unsigned {
+  COL_PRE_FFT = 1u << 0,
+  COL_PRE_IFFT = 1u << 1,
+  COL_MUL_P = 1u << 2,        // multiply by the propagator
+  COL_CONJ_P = 1u << 3,       // ... by its conjugate (adjoint)
+  COL_MUL_FILTER = 1u << 4,   // closed band [g2min, g2max] with 1/N (Filter_wave)
+  COL_POST_FFT = 1u << 5,
+  COL_POST_IFFT = 1u << 6,
+  COL_LOAD_PROBE = 1u << 7,   // value = A * ex[b][x] * ey[b][y]; nothing loaded
+  COL_LOAD_MUL_A = 1u << 8,   // value = wave * A
+  COL_STORE = 1u << 9,
+  COL_DET = 1u << 10,         // detector partial sums over closed bands
+};
+
+constexpr int kMaxDet = 8;
+
+struct ColArgs {
+  const float2* in;
+  float2* out;
+  long long pstride;
+  int nx, ny, nprobe, pb;
+  const float2* P;          // [ny][nx]
+  const float2* tw;         // [ny]
+  const float2* A;          // [ny][nx]
+  const float2* ex;         // [nprobe][nx]
+  const float2* ey;         // [nprobe][ny]
+  GridDev grid;
+  double fg2min, fg2max;    // filter band
+  float fscale;
+  int ndet;
+  double dg2min[kMaxDet], dg2max[kMaxDet];
+  float* dpart;             // [nprobe][ndet][ntiles]
+};
+
+hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
+hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
+// Raises the dynamic-LDS limit of every column-pass kernel of length ny
+// (call once per device before launching them).
+hipError_t prepare_col_kernels(int ny);
+bool fft_size_supported(int n);
+int row_lines_per_block(int nx);
+int col_tile(int ny);
+int col_ntiles(int nx, int ny);
+
+// --- other kernels ------------------------------------------------------------
+hipError_t launch_potential(const GridDev& g, int nslices, const int* slice_bin_off,
+                            int nbx, int nby, float bin_size, const float4* atoms,
+                            int ntypes, int nR, const float* tables, const float* rmax2,
+                            float* V, hipStream_t s);
+hipError_t launch_propagator(const GridDev& g, double w, double gx0, double gy0, int bwl,
+                             float2* P, hipStream_t s);
+hipError_t launch_lens_table(const GridDev& g, const LensDev& l, float2* A, hipStream_t s);
+hipError_t launch_probe_ramps(const GridDev& g, int nprobe, const double* xe, const double* ye,
+                              float2* ex, float2* ey, hipStream_t s);
+// out[d * ostride + b] (+)= w * sum_tiles part[b][d][tile]
+hipError_t launch_det_reduce(int nprobe, int ndet, int ntiles, const float* part, float w,
+                             int accumulate, float* out, long long ostride, hipStream_t s);
+hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g,
+                              hipStream_t s);
+hipError_t launch_fill(float2* p, long long n, float2 v, hipStream_t s);
+hipError_t launch_fft_shift(const float* in, float* out, int nx, int ny, long long nbatch, float w,
+                            int accumulate, hipStream_t s);
+hipError_t launch_abs2(const float2* in, float* out, long long n, float w, int accumulate,
+                       hipStream_t s);
+hipError_t launch_split(const float2* in, float* re, float* im, long long n, float w,
+                        int accumulate, hipStream_t s);
+hipError_t launch_grad_v(const float2* gt, const float* V, float sigma, long long n, float* gV,
+                         hipStream_t s);
+
+}  // namespace em

@@ -1,0 +1,213 @@
+// cudaEM/simulator.hpp -- the reference's C++ simulator surface, rebuilt on
+// the gfx950 engine (include/cudaem_amd.h).
+//
+// Names, argument meaning and call order follow the reference so that user
+// code written against it compiles and runs unchanged:
+//   cudaEM::TEM_simulator<T>        single_simulator.cuh:228-261
+//   cudaEM::scan_simulator<T>       scan_simulator.cuh:201-234
+//   cudaEM::Thickness_simulator<T>  thickness_simulator.cuh:87-119
+//   mulSimulator::execution{TEM,CBED,STEM}   simulator.cu:7-49
+// and the host structs the reference uses but does not ship (its
+// multislice_params.hpp, atom_coor_data.hpp, output_multislice_data.hpp are
+// missing): multislice_params<T>, Atom_Coord_Data<T>, Lens<T>, Detector<T>,
+// output_Multi_Data<T>, with the fields the reference reads
+// (multislice_params_dev.cuh:88-199, simulator.cu:9-46, lens_dev.cuh:74-131).
+//
+// Device work runs on one GPU (device 0): GPU_num larger than 1 runs every
+// frozen-phonon configuration there (the reference splits them over GPUs,
+// multislice_params_dev.cuh:92-97, and averages on the host).
+// Errors throw cudaEM::simulation_error (the reference prints and continues).
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace cudaEM {
+
+enum eTEM_Sim_Type { eTEMST_HRTEM = 1, eTEMST_ED = 2, eTEMST_CBED = 3, eTEMST_STEM = 4, eTEMST_Thickness = 5 };
+enum eIncident_Wave_Type { eIWT_Plane_Wave = 1, eIWT_Convergent_Wave = 2 };
+enum ePhonon_Model { ePM_Still_Atom = 1, ePM_Absorptive = 2, ePM_Frozen_Phonon = 3 };
+enum eIllumination_Model { eIM_Coherent = 1, eIM_Partial_Coherent = 2, eIM_Trans_Cross_Coef = 3 };
+enum eScanning_Type { eTile_scanning = 1, eGeneral_scanning = 2 };
+enum eData_Record { eIntensity = 1, eWave = 2 };
+enum eMatch_Border { eMB_Min = 0, eMB_Max = 1, eMB_Middle = 2 };
+
+struct simulation_error : std::runtime_error {
+  explicit simulation_error(const std::string& m) : std::runtime_error(m) {}
+};
+
+template <class T>
+struct Atom_Coord_Data {
+  std::vector<T> x, y, z, occ;
+  std::vector<int> Z;
+  T x_l = 0, y_l = 0;
+  std::size_t size() const { return x.size(); }
+  void push_back(int Zi, T xi, T yi, T zi, T occi = T(1)) {
+    Z.push_back(Zi);
+    x.push_back(xi);
+    y.push_back(yi);
+    z.push_back(zi);
+    occ.push_back(occi);
+  }
+};
+
+// Lens_Dev fields (lens_dev.cuh:12-66): aberrations in Angstrom, angles in
+// rad, apertures as semi-angles in mrad.
+template <class T>
+struct Lens {
+  int m = 0;
+  T c_10 = 0, c_12 = 0, phi_12 = 0, c_21 = 0, phi_21 = 0, c_23 = 0, phi_23 = 0;
+  T c_30 = 0, c_32 = 0, phi_32 = 0, c_34 = 0, phi_34 = 0;
+  T c_41 = 0, phi_41 = 0, c_43 = 0, phi_43 = 0, c_45 = 0, phi_45 = 0;
+  T c_50 = 0, c_52 = 0, phi_52 = 0, c_54 = 0, phi_54 = 0, c_56 = 0, phi_56 = 0;
+  T inner_aper_ang = 0, outer_aper_ang = 0;  // mrad
+};
+
+// Annular detectors; angles in mrad (wave_function.cuh:135-162 squares the
+// converted g_inner / g_outer).
+template <class T>
+struct Detector {
+  std::vector<T> ang_inner, ang_outer;
+  std::size_t size() const { return ang_inner.size(); }
+};
+
+template <class T>
+struct multislice_params {
+  eTEM_Sim_Type simulation_type = eTEMST_HRTEM;
+  int GPU_num = 1;
+  bool is_crystal = true;
+  // grid (Grid_2d_Dev): pixels, field of view, slice thickness, bwl
+  int nx = 0, ny = 0;
+  T lx = 0, ly = 0, dz = 0;
+  bool bwl = true;
+  // beam
+  T E_0 = 200;  // kV
+  T theta = 0, phi = 0;
+  eIncident_Wave_Type iw_type = eIWT_Plane_Wave;
+  T incident_wave_x_pos = 0, incident_wave_y_pos = 0;
+  eIllumination_Model illumination_model = eIM_Coherent;
+  // specimen
+  Atom_Coord_Data<T> atoms;
+  // phonons: Gaussian displacements of rms pn_sigma along enabled pn_dim
+  ePhonon_Model phonon_model = ePM_Still_Atom;
+  int pn_nconf = 1;
+  unsigned long long pn_seed = 1;
+  T pn_sigma = 0;
+  bool pn_dim[3] = {true, true, false};
+  // lenses and detectors
+  Lens<T> obj_lens, cond_lens;
+  Detector<T> detector;
+  eData_Record data_record = eIntensity;
+  // STEM scan: STEM_col x STEM_row positions over the normalised range
+  // [x0, y0, x1, y1] of the field (set_Scanning, simulator.cu:41-46)
+  eScanning_Type stem_scanning = eTile_scanning;
+  int STEM_col = 0, STEM_row = 0;
+  T scan_range[4] = {0, 0, 1, 1};
+  int probe_batch = 256;  // probes per batched launch (c_Probe_Batch)
+  // thickness series: frames every thickness_stride slices through
+  // detector thickness_det (the reference uses detector 1)
+  int thickness_stride = 1;
+  int thickness_det = 1;
+
+  void assign(const multislice_params& o) { *this = o; }
+  void validate_parameters() const;
+  void initial_OL() { iw_type = eIWT_Plane_Wave; }
+  void initial_CL() { iw_type = eIWT_Convergent_Wave; }
+  void set_Scanning(const std::vector<float>& r) {
+    if (r.size() != 4) throw simulation_error("scan range needs 4 values");
+    for (int i = 0; i < 4; ++i) scan_range[i] = (T)r[i];
+  }
+  bool frozen_phonon_enable() const { return phonon_model == ePM_Frozen_Phonon; }
+  bool scanning_enable() const { return simulation_type == eTEMST_STEM; }
+  int output_scan_pos_num() const { return STEM_col * STEM_row; }
+};
+
+template <class T>
+struct output_Multi_Data {
+  int row = 0, col = 0;
+  T interval_x = 0, interval_y = 0;
+  std::vector<T> image_intensity, image_real, image_imag;
+  std::vector<T> STEM_intensity;     // [iDet][y][x] (tile) or [iDet][ipos] (general)
+  std::vector<T> thickness_frames;   // [frame][pixel]
+  T total_intensity = 0;             // incident intensity, sum over g of |Psi_0(g)|^2
+  void init(int row_i, int col_i, int ndet, bool full_wave) {
+    row = row_i;
+    col = col_i;
+    if (ndet > 0) {
+      STEM_intensity.assign((std::size_t)row * col * ndet, T(0));
+    } else if (full_wave) {
+      image_real.assign((std::size_t)row * col, T(0));
+      image_imag.assign((std::size_t)row * col, T(0));
+    } else {
+      image_intensity.assign((std::size_t)row * col, T(0));
+    }
+  }
+};
+
+template <class T>
+class TEM_simulator {
+ public:
+  void set_input_data(multislice_params<T>* p, output_Multi_Data<T>* o) {
+    multisliceparam_host_ptr = p;
+    output_mulData_ptr = o;
+  }
+  void TEM_multiGPU();
+  output_Multi_Data<T>* output_mulData_ptr = nullptr;
+
+ private:
+  multislice_params<T>* multisliceparam_host_ptr = nullptr;
+};
+
+template <class T>
+class scan_simulator {
+ public:
+  void set_input_data(multislice_params<T>* p, output_Multi_Data<T>* o) {
+    multisliceparam_host_ptr = p;
+    output_mulData_ptr = o;
+  }
+  void STEM_multiGPU();
+  output_Multi_Data<T>* output_mulData_ptr = nullptr;
+
+ private:
+  multislice_params<T>* multisliceparam_host_ptr = nullptr;
+};
+
+template <class T>
+class Thickness_simulator {
+ public:
+  void set_input_data(multislice_params<T>* p, output_Multi_Data<T>* o) {
+    multisliceparam_host_ptr = p;
+    output_mulData_ptr = o;
+  }
+  void Thickness_multiGPU();
+  output_Multi_Data<T>* output_mulData_ptr = nullptr;
+
+ private:
+  multislice_params<T>* multisliceparam_host_ptr = nullptr;
+};
+
+// Per-device entry points of the reference (single_simulator.cuh:207,
+// scan_simulator.cuh:184, thickness_simulator.cuh:11).
+template <class T>
+void run_simu(multislice_params<T>* p, int GPU_ID, output_Multi_Data<T>* o);
+template <class T>
+void run_STEM_simu(multislice_params<T>* p, int GPU_ID, output_Multi_Data<T>* o);
+template <class T>
+void run_thickness_simu(multislice_params<T>* p, int GPU_ID, output_Multi_Data<T>* o);
+
+// Seeded frozen-phonon displacement (the reference's missing
+// vibrate_atom_positions): splitmix64 uniforms, Box-Muller normals consumed
+// x, y, z per atom, configuration iconf.
+template <class T>
+void vibrate_atom_positions(const Atom_Coord_Data<T>& in, Atom_Coord_Data<T>& out, unsigned long long seed,
+                            int iconf, T sigma, const bool dims[3]);
+
+}  // namespace cudaEM
+
+class mulSimulator {
+ public:
+  void executionTEM(const cudaEM::multislice_params<float>& in, cudaEM::output_Multi_Data<float>* out);
+  void executionCBED(const cudaEM::multislice_params<float>& in, cudaEM::output_Multi_Data<float>* out);
+  void executionSTEM(const cudaEM::multislice_params<float>& in, cudaEM::output_Multi_Data<float>* out);
+  void executionThickness(const cudaEM::multislice_params<float>& in, cudaEM::output_Multi_Data<float>* out);
+};

@@ -1,0 +1,220 @@
+/* cudaem_amd.h -- C ABI of the MI355X (gfx950) multislice engine.
+ *
+ * The reference (ningustc/cudaEM) exposes header-only C++ templates and no
+ * C ABI; its device work happens inside Wave_Function, Transmission_Function,
+ * Propagator, Incident_Wave and Lens_Modulation.  Each entry point below names
+ * the reference code it replaces (file:line, files of the reference's flat
+ * directory).  The C++ simulator surface (include/cudaEM/simulator.hpp) and
+ * the Python package (cudaem_amd/) are built on these entry points.
+ *
+ * Conventions
+ *  - Every function returns an em_status; nothing throws across the ABI.
+ *  - Pointers named *_dev are device pointers owned by the caller; all other
+ *    pointers are host pointers.  Complex device data is interleaved FP32
+ *    (float2), row-major [row][col] with col (x) fastest, batches outermost.
+ *  - Work is enqueued on the plan's stream (default: the null stream) and is
+ *    asynchronous unless the function returns host data.
+ *  - One plan per host thread or stream; a plan is not thread-safe.
+ *  - FFT convention: forward exp(-2 pi i nk/N), inverse exp(+2 pi i nk/N),
+ *    both unnormalised (fft_dev.cuh:233-264).  Exit waves are left in Fourier
+ *    space, unnormalised (wave_function.cuh:60-73).
+ */
+#ifndef CUDAEM_AMD_H
+#define CUDAEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int em_status;
+#define EM_OK 0
+#define EM_ERR_INVALID 1     /* bad argument or call order */
+#define EM_ERR_UNSUPPORTED 2 /* grid size / mode not built for gfx950 */
+#define EM_ERR_ALLOC 3       /* device or host allocation failed */
+#define EM_ERR_HIP 4         /* a HIP runtime call failed */
+
+typedef struct em_plan_s* em_plan;
+
+/* Simulation box: Grid_2d_Dev (grid_dev.cuh:10-56). */
+typedef struct em_grid {
+  int nx, ny;       /* columns (col) and rows (row); each of 64..2048 from the built set */
+  double lx, ly;    /* field of view, Angstrom (x_l, y_l) */
+  double dz;        /* slice thickness, Angstrom (dz) */
+  int bwl;          /* 1: 2/3 band-width limit in the propagator (bwl) */
+} em_grid;
+
+/* Beam: acceleration voltage and tilt (multislice_params_dev.cuh:40-41, 201-208). */
+typedef struct em_beam {
+  double e0_kv;
+  double theta, phi; /* rad */
+} em_beam;
+
+/* Lens: Lens_Dev (lens_dev.cuh:9-66), coefficients in Angstrom, angles in rad,
+ * apertures as semi-angles in mrad (g = sin(angle)/lambda; outer <= 0: none). */
+typedef struct em_lens {
+  double c_10, c_12, phi_12, c_21, phi_21, c_23, phi_23, c_30, c_32, phi_32, c_34, phi_34,
+      c_41, phi_41, c_43, phi_43, c_45, phi_45, c_50, c_52, phi_52, c_54, phi_54, c_56, phi_56;
+  double m;
+  double inner_aper_mrad, outer_aper_mrad;
+} em_lens;
+
+/* Atoms: Atom_Coord_Data (x, y, z, Z, occ); occ may be NULL (all 1). */
+typedef struct em_atoms {
+  long long n;
+  const double* x;
+  const double* y;
+  const double* z;
+  const int* Z;
+  const double* occ;
+} em_atoms;
+
+/* Slice alignment: eMatch_Border (slice_generator.hpp:137-166). */
+#define EM_BORDER_MIN 0    /* STEM / ED / CBED (specimen.cuh:33-36) */
+#define EM_BORDER_MAX 1    /* HRTEM (specimen.cuh:29-32) */
+#define EM_BORDER_MIDDLE 2
+
+/* ---- plan ------------------------------------------------------------------
+ * Replaces multislice_params_dev's device state and cuFFT plans
+ * (multislice_params_dev.cuh:84-199) and Propagator::update_prop_function
+ * (propagation_function.cuh:49-57, built once by wave_function.cuh:42-44).
+ * max_batch bounds the probes of one call to the batched entry points. */
+em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch, em_plan* out);
+em_status em_plan_destroy(em_plan plan);
+em_status em_plan_set_stream(em_plan plan, void* hip_stream);
+/* lambda (A), sigma (rad/(V A)), gamma; Vr_factor = sigma / cos(theta)
+ * (transmission_function.cuh:61-62). */
+em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double* gamma);
+/* Group of probes a fused pass keeps together (0 = automatic). */
+em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
+/* Last error message of this plan (static storage, valid until the next call). */
+const char* em_plan_last_error(em_plan plan);
+int em_fft_size_supported(int n);
+
+/* ---- specimen and potential ----------------------------------------------
+ * Element tables: the reference loads them in the missing element_coeffs_loader.hpp.
+ * Default: Kirkland's projected-potential form with Moliere coefficients per Z.
+ * em_set_element_coefs replaces them for one Z with Kirkland coefficients
+ * (a_i, b_i for the K0 terms, c_i, d_i for the Gaussian terms). */
+em_status em_set_element_coefs(em_plan plan, int Z, int nk, const double* a, const double* b,
+                               int ng, const double* c, const double* d);
+/* Spline nodes: nR nodes log-spaced in R over [r_min, r_max], cosine taper
+ * beyond tap_frac * r_max (defaults 128, 0.005, 3.0, 0.85). */
+em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max, double tap_frac);
+/* Copies the atoms, sorts them by z, slices them (Specimen::set_input_data,
+ * specimen.cuh:16-36 + Slice_Generator::calculate, slice_generator.hpp:48-168)
+ * and uploads the binned atom list.  Returns the slice count. */
+em_status em_plan_set_specimen(em_plan plan, const em_atoms* atoms, int border, int* n_slices);
+em_status em_slice_info(em_plan plan, int islice, double* z_start, double* z_end,
+                        long long* iatom_start, long long* iatom_end);
+/* Sigma-scaled projected potential of slices [s0, s0+ns) into V_dev[ns][ny][nx]:
+ * Projected_Potential::generate_potential_by_slice (projected_potential.cuh:47-92)
+ * with gen_slice_potential_kernel (device_kernels.cuh:28-47) and coeff = Vr_factor. */
+em_status em_potential_slices(em_plan plan, int s0, int ns, float* V_dev);
+/* Spline tables actually used (host copies, type order = sorted distinct Z):
+ * R2, c0..c3 each [ntypes][nR]; returns ntypes and nR. */
+em_status em_spline_tables(em_plan plan, int max_types, int* Zs, int* ntypes, int* nR,
+                           double* R2, double* c0, double* c1, double* c2, double* c3);
+
+/* Host-only helpers (no device work; usable without a GPU). */
+/* Slicing of z-sorted atom heights: Slice_Generator::gene_bounds / get_slicing /
+ * fd_by_z (slice_generator.hpp:81-168).  bounds[S+1], iatom_start[S],
+ * iatom_end[S] (inclusive; start > end for an empty slice); capacity = max_slices.
+ * Returns EM_ERR_INVALID when more than max_slices slices would be made. */
+em_status em_host_slicing(long long n, const double* z_sorted, double dz, int border, int max_slices,
+                          int* n_slices, double* bounds, long long* iatom_start, long long* iatom_end);
+/* Spline of the projected potential of element Z (default coefficients) in
+ * R^2: R2, c0..c3 each [nR] (basic_functions_dev.cuh:68-101 conventions). */
+em_status em_host_spline(int Z, int nR, double r_min, double r_max, double tap_frac, double* R2,
+                         double* c0, double* c1, double* c2, double* c3);
+/* lambda, gamma, sigma for an acceleration voltage (kV). */
+em_status em_host_constants(double e0_kv, double* lambda, double* gamma, double* sigma);
+
+/* ---- operators -------------------------------------------------------------*/
+/* Propagator table (gen_propagator_kernel, device_kernels.cuh:215-234). */
+em_status em_propagator(em_plan plan, void* P_dev /* float2[ny][nx] */);
+/* Probes in real space at (x[b], y[b]) with defocus c_10 + defocus_shift
+ * (Incident_Wave::generate_incident_beam(s), incident_wave.cuh:29-71). */
+em_status em_build_probes(em_plan plan, const em_lens* cond, double defocus_shift,
+                          const double* x, const double* y, int nprobe, void* psi_dev);
+/* Plane wave 1 (set_wave_val_kernel, incident_wave.cuh:45). */
+em_status em_plane_wave(em_plan plan, int nprobe, void* psi_dev);
+/* Unnormalised 2D FFT of nbatch planes, in place; dir -1 forward, +1 inverse
+ * (FFT_Dev::forward/inverse(_batch), fft_dev.cuh:159-190). */
+em_status em_fft2d(em_plan plan, void* data_dev, int nbatch, int dir);
+
+/* ---- multislice ------------------------------------------------------------
+ * Wave_Function::wave_modulation(_batch) (wave_function.cuh:60-87):
+ * psi_dev[B][N] in: incident wave (real space); out: exit wave (Fourier,
+ * unnormalised).  Exactly one of V_dev (sigma-scaled FP32 potential
+ * [S][N], t = exp(iV)) and t_dev (complex transmission float2[S][N]) is
+ * non-NULL.  ckpt_dev (optional, float2[S][B][N]) receives the wave entering
+ * each slice, for em_multislice_adjoint. */
+em_status em_multislice_forward(em_plan plan, const float* V_dev, const void* t_dev, int S,
+                                void* psi_dev, int nprobe, void* ckpt_dev);
+/* Adjoint of em_multislice_forward with respect to the transmission (no
+ * reference counterpart).  gPsi_dev[B][N] = dL/dRe(Psi) + i dL/dIm(Psi) of
+ * the exit waves; ckpt_dev from the forward call; gt_dev[S][N] receives
+ * dL/dRe(t) + i dL/dIm(t) summed over probes; gpsi0_dev (optional, [B][N])
+ * receives the gradient of the incident waves.  gPsi_dev is overwritten.
+ * With V_dev instead of t_dev, gV_dev (optional, FP32 [S][N]) receives
+ * dL/dV for the sigma-scaled potential as well. */
+em_status em_multislice_adjoint(em_plan plan, const float* V_dev, const void* t_dev, int S,
+                                const void* ckpt_dev, void* gPsi_dev, int nprobe, void* gt_dev,
+                                float* gV_dev, void* gpsi0_dev);
+
+/* Integrated detector intensities (integrated_intensity(_batch),
+ * wave_function.cuh:135-162; sum_square_over_Det(_batch),
+ * device_kernels.cuh:534-579, 625-671): out_dev[d][b] (+)= w * sum |Psi|^2
+ * over inner <= |g| <= outer (closed), detector angles in mrad. */
+em_status em_detector_sums(em_plan plan, const void* Psi_dev, int nprobe, int ndet,
+                           const double* inner_mrad, const double* outer_mrad, float w,
+                           int accumulate, float* out_dev);
+/* Coherent objective-lens image (Lens_Modulation::CTF_TEM,
+ * lens_modulation.cuh:78-89): I = |IFFT[Psi exp(i chi) A]|^2, or re/im when
+ * re_dev/im_dev are given (I_dev NULL); accumulate adds w * value. */
+em_status em_ctf_image(em_plan plan, const em_lens* obj, const void* Psi_dev, int nprobe,
+                       float* I_dev, float* re_dev, float* im_dev, float w, int accumulate);
+/* |Psi|^2 quadrant-shifted (ED / CBED output, single_simulator.cuh:103-110, 189). */
+em_status em_diffraction(em_plan plan, const void* Psi_dev, int nprobe, float* I_dev, float w,
+                         int accumulate);
+
+/* ---- simulations -----------------------------------------------------------
+ * Batched STEM over the current specimen (run_STEM_tile_simu's probe-batch
+ * loop, scan_simulator.cuh:87-109): potentials of every slice are built once,
+ * probes run in batches of max_batch through fused passes captured in a HIP
+ * graph, and detector sums land in out_dev[d][ipos] (+)= w * I. */
+em_status em_stem(em_plan plan, const em_lens* cond, int npos, const double* x, const double* y,
+                  int ndet, const double* inner_mrad, const double* outer_mrad, float w,
+                  int accumulate, float* out_dev);
+/* Thickness series (run_thickness_simu, thickness_simulator.cuh:11-85):
+ * after each slice the wave is band-passed by [g_inner, g_outer] (mrad) and
+ * |.|^2 recorded for slices with (k+1) % stride == 0 and the last one.
+ * frames_dev[nf][B][N] (+)= w * frame.  psi_dev: incident waves (real space),
+ * left holding the real-space wave after the last slice. */
+em_status em_thickness(em_plan plan, const float* V_dev, int S, void* psi_dev, int nprobe,
+                       double inner_mrad, double outer_mrad, int stride, float w, int accumulate,
+                       float* frames_dev, int* n_frames);
+
+/* Ragged batch: requests on different grids run as one HIP graph launch; all
+ * requests of one grid size share each fused pass.  Request r: plan plans[r]
+ * (its specimen set), psi_dev[r] incident waves [B_r][N_r] (overwritten),
+ * S[r] slices of V_dev[r], detector sums every `stride` slices into
+ * out_dev[r][nout_r][ndet][B_r] where nout_r = ceil(S_r/stride). */
+em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V_dev, const int* S,
+                               void* const* psi_dev, const int* nprobe, int ndet,
+                               const double* inner_mrad, const double* outer_mrad, int stride,
+                               float* const* out_dev, void* hip_stream);
+
+/* ---- measurement -----------------------------------------------------------
+ * No reference counterpart (the reference has no timing code).  Average
+ * device time in ms of the fused row pass (IFFT_x, exp(iV), FFT_x) and column
+ * pass (FFT_y, P, IFFT_y) over nprobe probes at wave_dev[B][N], each launched
+ * iters times between HIP events on the plan's stream.  V_dev: one slice
+ * [ny][nx].  The wave is overwritten. */
+em_status em_time_passes(em_plan plan, const float* V_dev, void* wave_dev, int nprobe, int iters,
+                         float* row_ms, float* col_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUDAEM_AMD_H */

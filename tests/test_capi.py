@@ -1,0 +1,121 @@
+"""CPU tests of the C ABI and the host logic: the library loads and exports
+every entry point of include/cudaem_amd.h, argument validation happens before
+any device work, and the host-side physics (slicing, splines, constants)
+matches the oracle."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cudaem_amd.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:em_status|int|const char\*)\s+(em_\w+)\(", txt, re.M)))
+
+
+@pytest.fixture(scope="module")
+def C():
+    from cudaem_amd import _capi
+    _capi.lib()
+    return _capi
+
+
+def test_library_exports_every_declared_entry_point(C):
+    names = header_functions()
+    assert len(names) >= 25
+    lib = ctypes.CDLL(C.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(C.EXPORTS) == names
+
+
+def test_lens_layout_is_shared():
+    from cudaem_amd import _capi
+    txt = open(HEADER).read()
+    body = re.search(r"typedef struct em_lens \{(.*?)\} em_lens;", txt, re.S).group(1)
+    fields = re.findall(r"\b(c_\d\d|phi_\d\d|m|inner_aper_mrad|outer_aper_mrad)\b", body)
+    assert fields == _capi.LENS_FIELDS == O.LENS_FIELDS
+
+
+def test_fft_sizes(C):
+    for n in (64, 128, 256, 512, 768, 1024, 2048):
+        assert C.lib().em_fft_size_supported(n) == 1
+    for n in (0, 100, 1000, 4096):
+        assert C.lib().em_fft_size_supported(n) == 0
+
+
+def test_validation_before_device_work(C):
+    L = C.lib()
+    h = ctypes.c_void_p()
+    assert L.em_plan_create(None, None, 1, ctypes.byref(h)) == C.EM_ERR_INVALID
+    g = C.em_grid(100, 100, 10.0, 10.0, 1.0, 1)
+    b = C.em_beam(200.0, 0.0, 0.0)
+    assert L.em_plan_create(ctypes.byref(g), ctypes.byref(b), 1, ctypes.byref(h)) == C.EM_ERR_UNSUPPORTED
+    g = C.em_grid(128, 128, -1.0, 10.0, 1.0, 1)
+    assert L.em_plan_create(ctypes.byref(g), ctypes.byref(b), 1, ctypes.byref(h)) == C.EM_ERR_INVALID
+    g = C.em_grid(128, 128, 10.0, 10.0, 1.0, 1)
+    assert L.em_plan_create(ctypes.byref(g), ctypes.byref(b), 0, ctypes.byref(h)) == C.EM_ERR_INVALID
+    assert L.em_plan_destroy(None) == C.EM_ERR_INVALID
+    assert L.em_stem(None, None, 0, None, None, 0, None, None, 1.0, 0, None) == C.EM_ERR_INVALID
+    assert L.em_multislice_forward(None, None, None, 0, None, 0, None) == C.EM_ERR_INVALID
+    assert L.em_multislice_ragged(0, None, None, None, None, None, 0, None, None, 1, None, None) == C.EM_ERR_INVALID
+
+
+def test_host_constants_match_oracle(C):
+    for kv in (80.0, 200.0, 300.0):
+        l, g, s = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        assert C.lib().em_host_constants(kv, ctypes.byref(l), ctypes.byref(g), ctypes.byref(s)) == 0
+        lo, go, so = O.constants(kv)
+        assert (l.value, g.value, s.value) == (lo, go, so)
+
+
+def _host_slicing(C, z, dz, border):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    cap = 4096
+    ns = ctypes.c_int()
+    b = np.zeros(cap + 1)
+    i0 = np.zeros(cap, np.int64)
+    i1 = np.zeros(cap, np.int64)
+    st = C.lib().em_host_slicing(len(z), z.ctypes.data_as(C._PD), dz, border, cap, ctypes.byref(ns),
+                                 b.ctypes.data_as(C._PD), i0.ctypes.data_as(C._PLL), i1.ctypes.data_as(C._PLL))
+    assert st == 0
+    S = ns.value
+    return b[:S + 1], i0[:S], i1[:S]
+
+
+@pytest.mark.parametrize("border", [0, 1, 2])
+def test_host_slicing_matches_oracle(C, border):
+    import cudaem_amd.specimen as sp
+    rng = np.random.default_rng(border)
+    cases = [np.sort(rng.uniform(-5, 30, 700)), np.sort(sp.fcc_cu(3, 3, 10)["z"]),
+             np.sort(sp.vibrate(sp.srtio3(2, 2, 40), 0.08, 5, 1)["z"]), np.array([1.0])]
+    for z in cases:
+        for dz in (1.8075, 1.9525, 0.7):
+            b, i0, i1 = _host_slicing(C, z, dz, border)
+            bo, i0o, i1o = O.slicing(z, dz, border)
+            assert np.array_equal(b, bo) and np.array_equal(i0, i0o) and np.array_equal(i1, i1o)
+
+
+def test_host_slicing_rejects_unsorted(C):
+    z = np.array([1.0, 0.5])
+    ns = ctypes.c_int()
+    assert C.lib().em_host_slicing(2, z.ctypes.data_as(C._PD), 1.0, 0, 10, ctypes.byref(ns), None, None,
+                                   None) == C.EM_ERR_INVALID
+
+
+@pytest.mark.parametrize("Z", [8, 14, 22, 29, 38, 79])
+def test_host_spline_matches_oracle(C, Z):
+    nR = 128
+    arrs = [np.zeros(nR) for _ in range(5)]
+    assert C.lib().em_host_spline(Z, nR, O.SPLINE_RMIN, O.SPLINE_RMAX, O.SPLINE_TAP,
+                                  *[a.ctypes.data_as(C._PD) for a in arrs]) == 0
+    ref = O.build_spline(Z)
+    for got, exp in zip(arrs, ref):
+        np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())

@@ -1,0 +1,280 @@
+"""GPU parity of the HIP engine against the FP64 oracle (same seeded inputs).
+
+Tolerances (relative L2, from the engine's accuracy contract): 1e-4 on
+exit waves, 1e-3 on detector intensities and images, 1e-5 on single FFTs,
+tables and potentials.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import OracleSpecimen, probe_lens, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_WAVE = 1e-4
+TOL_INT = 1e-3
+TOL_OP = 1e-5
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def em(cuda):
+    import cudaem_amd
+    return cudaem_amd
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 2048])
+def test_fft2d_matches_oracle(em, n):
+    import torch
+    rng = np.random.default_rng(n)
+    B = 2 if n <= 1024 else 1
+    x = (rng.standard_normal((B, n, n)) + 1j * rng.standard_normal((B, n, n))).astype(np.complex64)
+    print(f"fft2d n={n} B={B}", flush=True)
+    with em.Plan(n, n, 10.0, 10.0, 1.0, 200.0, max_batch=B) as p:
+        d = torch.from_numpy(x).cuda()
+        p.fft2d(d, -1)
+        fwd = _np(d)
+        p.fft2d(d, +1)
+        back = _np(d)
+    for b in range(B):
+        ref = np.fft.fft2(x[b].astype(np.complex128))
+        assert rel_l2(fwd[b], ref) < TOL_OP
+        assert rel_l2(back[b], x[b] * (n * n)) < TOL_OP
+    if n <= 256:  # the oracle FFT itself on the same input
+        assert rel_l2(fwd[0], O.fft2d(x[0])) < TOL_OP
+
+
+def test_fft2d_non_square(em):
+    import torch
+    rng = np.random.default_rng(7)
+    nx, ny = 256, 128
+    x = (rng.standard_normal((3, ny, nx)) + 1j * rng.standard_normal((3, ny, nx))).astype(np.complex64)
+    with em.Plan(nx, ny, 20.0, 10.0, 1.0, 200.0, max_batch=3) as p:
+        d = torch.from_numpy(x).cuda()
+        p.fft2d(d, -1)
+        out = _np(d)
+    for b in range(3):
+        assert rel_l2(out[b], np.fft.fft2(x[b].astype(np.complex128))) < TOL_OP
+
+
+@pytest.mark.parametrize("bwl", [True, False])
+def test_propagator_matches_oracle(em, bwl):
+    n, L, dz = 256, 28.92, 1.8075
+    with em.Plan(n, n, L, L, dz, 200.0, bwl=bwl) as p:
+        P = _np(p.propagator())
+        lam = p.lam
+    ref = O.propagator(n, n, L, L, dz, bwl, lam)
+    assert rel_l2(P, ref) < TOL_OP
+
+
+def test_potential_matches_oracle(em):
+    import cudaem_amd.specimen as sp
+    import torch
+    at = sp.fcc_cu(4, 4, 3)
+    n = 128
+    with em.Plan(n, n, at["lx"], at["ly"], 1.8075, 200.0) as p:
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MAX)
+        V = _np(p.potential_slices())
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], 1.8075, 200.0, O.BORDER_MAX)
+    assert S == o.S == 6
+    Vo = o.potentials()
+    assert rel_l2(V, Vo) < TOL_OP
+
+
+def test_spline_tables_match_oracle(em):
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(2, 2, 2)
+    with em.Plan(64, 64, at["lx"], at["ly"], 1.9525, 300.0) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+        Zs, (R2, c0, c1, c2, c3) = p.spline_tables()
+    assert list(Zs) == [8, 22, 38]
+    for t, Z in enumerate(Zs):
+        ref = O.build_spline(int(Z))
+        for got, exp in zip((R2[t], c0[t], c1[t], c2[t], c3[t]), ref):
+            np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+
+
+def test_probe_matches_oracle(em):
+    n, L = 256, 20.0
+    lens = {"c_10": 35.0, "c_30": 1.2e4, "outer_aper_mrad": 21.0, "c_12": 8.0, "phi_12": 0.3}
+    xs = np.array([3.0, 10.0, 17.5])
+    ys = np.array([5.0, 10.0, 2.25])
+    with em.Plan(n, n, L, L, 2.0, 300.0, max_batch=3) as p:
+        psi = _np(p.build_probes(lens, xs, ys, defocus_shift=0.75))
+        lam = p.lam
+    for b in range(3):
+        ref = O.probe(n, n, L, L, lam, lens, xs[b], ys[b], defocus_shift=0.75)
+        assert rel_l2(psi[b], ref) < TOL_OP
+
+
+def _hrtem_config1():
+    import cudaem_amd.specimen as sp
+    at = sp.fcc_cu(8, 8, 10)
+    return at, 256, 1.8075, 200.0
+
+
+def test_hrtem_config1_exit_wave_and_ctf(em):
+    """Plain HRTEM plane-wave multislice: 256^2, FCC Cu 8x8x10, 20 slices,
+    200 kV; exit wave and Cs = 1 mm / defocus -50 nm CTF image."""
+    import torch
+    at, n, dz, e0 = _hrtem_config1()
+    obj = {"c_30": 1.0e7, "c_10": -500.0}
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0) as p:
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MAX)
+        V = p.potential_slices()
+        psi = p.plane_wave(1)
+        p.multislice_forward(psi, V=V)
+        Psi = _np(psi)[0]
+        I = _np(p.ctf_image(obj, psi))[0]
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MAX)
+    assert S == o.S == 20
+    Vo = o.potentials()
+    Po = o.propagator()
+    Psi_o = O.multislice(Po, np.ones((n, n), complex), V=Vo)
+    I_o, _ = O.ctf_image(Psi_o, at["lx"], at["ly"], o.lam, obj)
+    e_wave = rel_l2(Psi, Psi_o)
+    e_img = rel_l2(I, I_o)
+    print(f"config1 exit-wave relL2={e_wave:.3e} ctf-image relL2={e_img:.3e}")
+    assert e_wave < TOL_WAVE
+    assert e_img < TOL_INT
+
+
+def _stem_setup(n=256, cells=4, nz=6):
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(cells, cells, nz)
+    return at, n, 1.9525, 300.0
+
+
+def test_stem_probes_exit_waves_and_detectors(em):
+    import torch
+    at, n, dz, e0 = _stem_setup()
+    lens = probe_lens(25.0)
+    xs = np.array([1.0, 4.2, 7.9, 12.3])
+    ys = np.array([0.5, 6.6, 3.3, 14.0])
+    inner, outer = np.array([60.0, 20.0]), np.array([200.0, 60.0])
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=4) as p:
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        zs, ze, _, _ = p.slice_info(0)
+        V = p.potential_slices()
+        psi = p.build_probes(lens, xs, ys, defocus_shift=0.5 * (zs + ze))
+        p.multislice_forward(psi, V=V)
+        det = _np(p.detector_sums(psi, inner, outer))
+        stem = _np(p.stem(lens, xs, ys, inner, outer))
+        Psi = _np(psi)
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
+    assert S == o.S
+    Vo = o.potentials()
+    Po = o.propagator()
+    g2 = [(np.sin(a * 1e-3) / o.lam) ** 2 for a in (inner, outer)]
+    ref_det, _ = O.stem(n, n, at["lx"], at["ly"], o.lam, lens, o.zmid0(), Po, Vo, xs, ys, g2[0], g2[1])
+    for b in range(len(xs)):
+        psi0 = O.probe(n, n, at["lx"], at["ly"], o.lam, lens, xs[b], ys[b], o.zmid0())
+        Psi_o = O.multislice(Po, psi0, V=Vo)
+        assert rel_l2(Psi[b], Psi_o) < TOL_WAVE
+    print("detector relL2", rel_l2(det, ref_det), rel_l2(stem, ref_det))
+    assert rel_l2(det, ref_det) < TOL_INT
+    assert rel_l2(stem, ref_det) < TOL_INT
+
+
+def test_stem_batches_and_groups_agree(em):
+    """em_stem with several batches / probe groups equals one big batch."""
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 4)
+    n = 128
+    xs, ys = sp.raster(5, 5, at["lx"], at["ly"])
+    lens = probe_lens(20.0)
+    res = []
+    for mb, grp in ((25, 0), (8, 3), (1, 1)):
+        with em.Plan(n, n, at["lx"], at["ly"], 1.9525, 300.0, max_batch=mb) as p:
+            p.set_probe_group(grp)
+            p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+            res.append(_np(p.stem(lens, xs, ys, [40.0], [120.0])))
+    assert rel_l2(res[1], res[0]) < 1e-6
+    assert rel_l2(res[2], res[0]) < 1e-6
+
+
+def test_thickness_series_matches_oracle(em):
+    import torch
+    at, n, dz, e0 = _stem_setup(n=128, cells=3, nz=4)
+    lens = probe_lens(20.0)
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=2) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+        zs, ze, _, _ = p.slice_info(0)
+        V = p.potential_slices()
+        psi = p.build_probes(lens, [2.0, 5.0], [3.0, 1.0], defocus_shift=0.5 * (zs + ze))
+        psi0 = _np(psi)
+        frames = _np(p.thickness(V, psi, 0.0, 40.0, stride=3))
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
+    Vo, Po = o.potentials(), o.propagator()
+    g2min, g2max = 0.0, (np.sin(40e-3) / o.lam) ** 2
+    for b in range(2):
+        fo, _ = O.thickness(Po, Vo, psi0[b].astype(np.complex128), at["lx"], at["ly"], g2min, g2max, stride=3)
+        assert frames.shape[0] == fo.shape[0]
+        assert rel_l2(frames[:, b], fo) < TOL_INT
+
+
+def test_adjoint_matches_oracle(em):
+    import torch
+    rng = np.random.default_rng(5)
+    n, S, B = 128, 5, 3
+    L = 15.0
+    t = (rng.uniform(0.95, 1.0, (S, n, n)) * np.exp(1j * rng.uniform(0, 0.5, (S, n, n))))
+    with em.Plan(n, n, L, L, 2.0, 300.0, max_batch=B) as p:
+        psi = p.build_probes(probe_lens(20.0), rng.uniform(0, L, B), rng.uniform(0, L, B))
+        psi0 = _np(psi).astype(np.complex128)
+        tt = torch.from_numpy(t.astype(np.complex64)).cuda()
+        ck = torch.empty((S, B, n, n), dtype=torch.complex64, device="cuda")
+        p.multislice_forward(psi, t=tt, ckpt=ck)
+        Psi = _np(psi).astype(np.complex128)
+        target = rng.uniform(0.5, 1.5, (B, n, n)) * np.abs(Psi) ** 2
+        gPsi = 4 * (np.abs(Psi) ** 2 - target) * Psi
+        g = torch.from_numpy(gPsi.astype(np.complex64)).cuda()
+        gt = _np(p.multislice_adjoint(ck, g, t=tt))
+        P = _np(p.propagator()).astype(np.complex128)
+    gt_o, Psi_o = O.adjoint(P, t, psi0, gPsi)
+    assert rel_l2(Psi, Psi_o) < TOL_WAVE
+    e = rel_l2(gt, gt_o)
+    print("adjoint relL2", e)
+    assert e < TOL_INT
+
+
+def test_ragged_matches_separate_runs(em):
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.si_diamond(3, 3, 3)
+    reqs = [(128, 300.0, 2), (256, 200.0, 3), (64, 300.0, 1)]
+    plans, Vs, psis, refs = [], [], [], []
+    for n, e0, B in reqs:
+        p = em.Plan(n, n, at["lx"], at["ly"], 1.35775, e0, max_batch=B)
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+        V = p.potential_slices()
+        xs = np.linspace(1, at["lx"] - 1, B)
+        psi = p.build_probes(probe_lens(15.0), xs, xs[::-1].copy())
+        ref_psi = psi.clone()
+        # reference: the same slices one by one through the plain path
+        S = V.shape[0]
+        outs = []
+        for s in range(S):
+            if (s + 1) % 4 == 0 or s == S - 1:
+                w = ref_psi.clone()
+                p.multislice_forward(w, V=V[: s + 1].contiguous())
+                outs.append(_np(p.detector_sums(w, [10.0, 30.0], [30.0, 80.0])))
+        refs.append(np.stack(outs))
+        plans.append(p)
+        Vs.append(V)
+        psis.append(psi)
+    got = em.ragged(plans, Vs, psis, [10.0, 30.0], [30.0, 80.0], stride=4)
+    torch.cuda.synchronize()
+    for g, r in zip(got, refs):
+        assert g.shape == r.shape
+        assert rel_l2(_np(g), r) < 1e-5
+    for p in plans:
+        p.close()
