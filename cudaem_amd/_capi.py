@@ -83,8 +83,8 @@ _SIGS = {
     "em_build_probes": ([_P, ctypes.POINTER(em_lens), _D, _PD, _PD, _I, _P], _I),
     "em_plane_wave": ([_P, _I, _P], _I),
     "em_fft2d": ([_P, _P, _I, _I], _I),
-    "em_multislice_forward": ([_P, _P, _P, _I, _P, _I, _P], _I),
-    "em_multislice_adjoint": ([_P, _P, _P, _I, _P, _P, _I, _P, _P, _P], _I),
+    "em_multislice_forward": ([_P, _P, _P, _I, _P, _I, _P, _I], _I),
+    "em_multislice_adjoint": ([_P, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _I),
     "em_detector_sums": ([_P, _P, _I, _I, _PD, _PD, _F, _I, _P], _I),
     "em_ctf_image": ([_P, ctypes.POINTER(em_lens), _P, _I, _P, _P, _P, _F, _I], _I),
     "em_diffraction": ([_P, _P, _I, _P, _F, _I], _I),

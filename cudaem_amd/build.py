@@ -45,7 +45,29 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_test(verbose)
     return LIB
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_simulators.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_simulators")
+
+
+def build_cpp_test(verbose=False):
+    """The C++ drop-in test program: plain host C++ against the simulator
+    surface, linked to the library (rpath relative to the binary)."""
+    if not os.path.exists(CPP_TEST_SRC):
+        return None
+    tmp = CPP_TEST_BIN + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC, "-o", tmp,
+           "-L", HERE, "-lcudaem_amd", "-Wl,-rpath,$ORIGIN/../../cudaem_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test program failed to build:\n{r.stderr}")
+    os.replace(tmp, CPP_TEST_BIN)
+    if verbose:
+        print("built", CPP_TEST_BIN)
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -87,7 +87,7 @@ struct em_plan_s {
   float bin = 0;
   // device
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
-  DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, detout;
+  DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, detout, seg, segwave;
   int vstore_slices = -1;
   std::map<GraphKey, std::unique_ptr<Graph>> graphs;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
@@ -183,7 +183,7 @@ em_status ensure_specimen_potential(em_plan p);
 
 // Fused forward multislice over nb probes at wave (real space in, Fourier out).
 em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
-                        float2* ckpt, long long ckpt_slice_stride, bool in_mixed_layout) {
+                        float2* ckpt, long long ckpt_slice_stride, int every, bool in_mixed_layout) {
   for (int s = 0; s < S; ++s) {
     RowArgs r = row_args(p, wave, wave, nb);
     unsigned m = ROW_POST_FFT;
@@ -195,9 +195,9 @@ em_status forward_group(em_plan p, const float* V, const float2* t, int S, float
       m |= ROW_MUL_T;
       r.t = t + (long long)s * p->N();
     }
-    if (ckpt) {
+    if (ckpt && s % every == 0) {
       m |= ROW_CKPT_STORE;
-      r.ckpt = ckpt + (long long)s * ckpt_slice_stride;
+      r.ckpt = ckpt + (long long)(s / every) * ckpt_slice_stride;
     }
     EM_TRY(row(p, m, r));
     ColArgs c = col_args(p, wave, wave, nb);
@@ -522,23 +522,24 @@ em_status em_fft2d(em_plan p, void* data, int nbatch, int dir) {
 }
 
 em_status em_multislice_forward(em_plan p, const float* V, const void* t, int S, void* psi, int nprobe,
-                                void* ckpt) {
-  if (!p || !psi || S <= 0 || nprobe <= 0 || (!V) == (!t)) return EM_ERR_INVALID;
+                                void* ckpt, int every) {
+  if (!p || !psi || S <= 0 || nprobe <= 0 || (!V) == (!t) || every <= 0) return EM_ERR_INVALID;
   float2* w = reinterpret_cast<float2*>(psi);
   const int G = group_size(p, nprobe);
   for (int g0 = 0; g0 < nprobe; g0 += G) {
     const int nb = std::min(G, nprobe - g0);
     float2* ck = ckpt ? reinterpret_cast<float2*>(ckpt) + (long long)g0 * p->N() : nullptr;
     EM_TRY(forward_group(p, V, reinterpret_cast<const float2*>(t), S, w + (long long)g0 * p->N(), nb, ck,
-                         (long long)nprobe * p->N(), false));
+                         (long long)nprobe * p->N(), every, false));
   }
   return EM_OK;
 }
 
-em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S, const void* ckpt,
+em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S, const void* ckpt, int every,
                                 void* gPsi, int nprobe, void* gt, float* gV, void* gpsi0) {
-  if (!p || !ckpt || !gPsi || !gt || S <= 0 || nprobe <= 0 || (!V) == (!t)) return EM_ERR_INVALID;
+  if (!p || !ckpt || !gPsi || !gt || S <= 0 || nprobe <= 0 || (!V) == (!t) || every <= 0) return EM_ERR_INVALID;
   const long long N = p->N();
+  const long long BN = (long long)nprobe * N;
   float2* lam = reinterpret_cast<float2*>(gPsi);
   const float2* ck = reinterpret_cast<const float2*>(ckpt);
   const float2* tc = reinterpret_cast<const float2*>(t);
@@ -546,11 +547,40 @@ em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S,
   const int pb = row_pb(p, nprobe);
   const int nparts = (nprobe + pb - 1) / pb;
   EM_HIP(p, p->gpart.ensure(sizeof(float2) * N * nparts));
+  if (every > 1) {
+    // segment workspace: the recomputed waves entering each slice of a segment
+    EM_HIP(p, p->seg.ensure(sizeof(float2) * BN * every));
+    EM_HIP(p, p->segwave.ensure(sizeof(float2) * BN));
+  }
   // lambda_{S-1} = FFT^H gPsi: column inverse now, row inverse inside the first backward row pass
   EM_TRY(col(p, COL_PRE_IFFT | COL_STORE, col_args(p, lam, lam, nprobe)));
   for (int k = S - 1; k >= 0; --k) {
+    const float2* ckk;
+    if (every == 1) {
+      ckk = ck + (long long)k * BN;
+    } else {
+      const int j = k / every, s0 = j * every;
+      if (k == S - 1 || k % every == every - 1) {
+        // recompute the waves entering slices s0..k from checkpoint j
+        float2* sw = p->segwave.as<float2>();
+        float2* sg = p->seg.as<float2>();
+        EM_HIP(p, hipMemcpyAsync(sw, ck + (long long)j * BN, sizeof(float2) * BN, hipMemcpyDeviceToDevice, p->stream));
+        for (int s = s0; s <= k; ++s) {
+          RowArgs r = row_args(p, sw, sw, nprobe);
+          r.ckpt = sg + (long long)(s - s0) * BN;
+          unsigned m = ROW_CKPT_STORE | (s > s0 ? ROW_PRE_IFFT : 0u);
+          if (s < k) {
+            m |= ROW_POST_FFT | (V ? ROW_MUL_V : ROW_MUL_T);
+            if (V) r.V = V + (long long)s * N; else r.t = tc + (long long)s * N;
+          }
+          EM_TRY(row(p, m, r));
+          if (s < k) EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, sw, sw, nprobe)));
+        }
+      }
+      ckk = p->seg.as<float2>() + (long long)(k - s0) * BN;
+    }
     RowArgs r = row_args(p, lam, lam, nprobe);
-    r.ckpt = const_cast<float2*>(ck) + (long long)k * nprobe * N;
+    r.ckpt = const_cast<float2*>(ckk);
     r.gpart = nparts == 1 ? gtc + (long long)k * N : p->gpart.as<float2>();
     unsigned m = ROW_PRE_IFFT | ROW_GRAD;
     if (k > 0 || gpsi0) {

@@ -281,6 +281,8 @@ __global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a
   X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_V | ROW_CONJ)                             \
   X(ROW_PRE_IFFT | ROW_CKPT_STORE | ROW_MUL_V | ROW_POST_FFT)                   \
   X(ROW_CKPT_STORE | ROW_MUL_V | ROW_POST_FFT)                                  \
+  X(ROW_CKPT_STORE)                                                             \
+  X(ROW_PRE_IFFT | ROW_CKPT_STORE)                                              \
   X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_V | ROW_CONJ | ROW_POST_FFT)              \
   X(ROW_PRE_IFFT | ROW_OUT_ABS2)                                                \
   X(ROW_PRE_IFFT | ROW_OUT_ABS2 | ROW_OUT_ACCUM)                                \

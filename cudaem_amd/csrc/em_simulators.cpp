@@ -250,7 +250,7 @@ void run_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T>* o)
     } else {
       ok(em_plane_wave(plan.p, 1, psi.p), plan.p, "em_plane_wave");
     }
-    ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr), plan.p, "em_multislice_forward");
+    ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr, 1), plan.p, "em_multislice_forward");
     if (m.simulation_type == eTEMST_HRTEM) {
       if (full)
         ok(em_ctf_image(plan.p, &obj, psi.p, 1, nullptr, a.as<float>(), b.as<float>(), w, 1), plan.p, "em_ctf_image");
@@ -340,7 +340,7 @@ void run_STEM_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T
         ok(em_potential_slices(plan.p, 0, S, V.as<float>()), plan.p, "em_potential_slices");
         const double xb = 0, yb = 0;
         ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, psi.p), plan.p, "em_build_probes");
-        ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr), plan.p,
+        ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr, 1), plan.p,
            "em_multislice_forward");
         ok(em_detector_sums(plan.p, psi.p, 1, nd, in.data(), ou.data(), w, 0, d.as<float>()), plan.p,
            "em_detector_sums");

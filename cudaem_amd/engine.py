@@ -187,10 +187,18 @@ class Plan:
         self._call("em_fft2d", _ptr(data), B, int(direction))
         return data
 
-    def multislice_forward(self, psi, V=None, t=None, ckpt=None):
+    @staticmethod
+    def n_checkpoints(S, every):
+        return (S + every - 1) // every
+
+    def multislice_forward(self, psi, V=None, t=None, ckpt=None, every=1):
+        """psi: incident waves [B][ny][nx] (real space) -> exit waves (Fourier,
+        unnormalised), in place.  ckpt: optional [ceil(S/every)][B][ny][nx]."""
         torch = _torch()
         if (V is None) == (t is None):
             raise ValueError("give exactly one of V and t")
+        if every < 1:
+            raise ValueError("every >= 1")
         B = psi.shape[0]
         _need(psi, (B, self.ny, self.nx), torch.complex64, "psi")
         S = (V if V is not None else t).shape[0]
@@ -199,18 +207,20 @@ class Plan:
         else:
             _need(t, (S, self.ny, self.nx), torch.complex64, "t")
         if ckpt is not None:
-            _need(ckpt, (S, B, self.ny, self.nx), torch.complex64, "ckpt")
-        self._call("em_multislice_forward", _ptr(V), _ptr(t), S, _ptr(psi), B, _ptr(ckpt))
+            _need(ckpt, (self.n_checkpoints(S, every), B, self.ny, self.nx), torch.complex64, "ckpt")
+        self._call("em_multislice_forward", _ptr(V), _ptr(t), S, _ptr(psi), B, _ptr(ckpt), int(every))
         return psi
 
-    def multislice_adjoint(self, ckpt, gPsi, V=None, t=None, gt=None, gV=None, gpsi0=None):
+    def multislice_adjoint(self, ckpt, gPsi, V=None, t=None, gt=None, gV=None, gpsi0=None, every=1):
+        """Gradient w.r.t. the transmission of a loss whose exit-wave gradient is
+        gPsi (dL/dRe + i dL/dIm; overwritten).  Returns gt [S][ny][nx]."""
         torch = _torch()
         if (V is None) == (t is None):
             raise ValueError("give exactly one of V and t")
         S = (V if V is not None else t).shape[0]
         B = gPsi.shape[0]
         _need(gPsi, (B, self.ny, self.nx), torch.complex64, "gPsi")
-        _need(ckpt, (S, B, self.ny, self.nx), torch.complex64, "ckpt")
+        _need(ckpt, (self.n_checkpoints(S, every), B, self.ny, self.nx), torch.complex64, "ckpt")
         if V is not None:
             _need(V, (S, self.ny, self.nx), torch.float32, "V")
         else:
@@ -222,7 +232,7 @@ class Plan:
             _need(gV, (S, self.ny, self.nx), torch.float32, "gV")
         if gpsi0 is not None:
             _need(gpsi0, (B, self.ny, self.nx), torch.complex64, "gpsi0")
-        self._call("em_multislice_adjoint", _ptr(V), _ptr(t), S, _ptr(ckpt), _ptr(gPsi), B, _ptr(gt),
+        self._call("em_multislice_adjoint", _ptr(V), _ptr(t), S, _ptr(ckpt), int(every), _ptr(gPsi), B, _ptr(gt),
                    _ptr(gV), _ptr(gpsi0))
         return gt
 

@@ -147,20 +147,22 @@ em_status em_fft2d(em_plan plan, void* data_dev, int nbatch, int dir);
  * psi_dev[B][N] in: incident wave (real space); out: exit wave (Fourier,
  * unnormalised).  Exactly one of V_dev (sigma-scaled FP32 potential
  * [S][N], t = exp(iV)) and t_dev (complex transmission float2[S][N]) is
- * non-NULL.  ckpt_dev (optional, float2[S][B][N]) receives the wave entering
- * each slice, for em_multislice_adjoint. */
+ * non-NULL.  ckpt_dev (optional, float2[ceil(S/ckpt_every)][B][N]) receives
+ * the wave entering slices 0, ckpt_every, 2*ckpt_every, ... (real space), for
+ * em_multislice_adjoint; ckpt_every = 1 keeps every slice. */
 em_status em_multislice_forward(em_plan plan, const float* V_dev, const void* t_dev, int S,
-                                void* psi_dev, int nprobe, void* ckpt_dev);
+                                void* psi_dev, int nprobe, void* ckpt_dev, int ckpt_every);
 /* Adjoint of em_multislice_forward with respect to the transmission (no
  * reference counterpart).  gPsi_dev[B][N] = dL/dRe(Psi) + i dL/dIm(Psi) of
- * the exit waves; ckpt_dev from the forward call; gt_dev[S][N] receives
- * dL/dRe(t) + i dL/dIm(t) summed over probes; gpsi0_dev (optional, [B][N])
- * receives the gradient of the incident waves.  gPsi_dev is overwritten.
- * With V_dev instead of t_dev, gV_dev (optional, FP32 [S][N]) receives
- * dL/dV for the sigma-scaled potential as well. */
+ * the exit waves (overwritten); ckpt_dev and ckpt_every as given to the
+ * forward call: waves between checkpoints are recomputed segment by segment
+ * into plan-owned workspace.  gt_dev[S][N] receives dL/dRe(t) + i dL/dIm(t)
+ * summed over probes; gpsi0_dev (optional, [B][N]) the gradient of the
+ * incident waves.  With V_dev, gV_dev (optional, FP32 [S][N]) receives dL/dV
+ * of the sigma-scaled potential. */
 em_status em_multislice_adjoint(em_plan plan, const float* V_dev, const void* t_dev, int S,
-                                const void* ckpt_dev, void* gPsi_dev, int nprobe, void* gt_dev,
-                                float* gV_dev, void* gpsi0_dev);
+                                const void* ckpt_dev, int ckpt_every, void* gPsi_dev, int nprobe,
+                                void* gt_dev, float* gV_dev, void* gpsi0_dev);
 
 /* Integrated detector intensities (integrated_intensity(_batch),
  * wave_function.cuh:135-162; sum_square_over_Det(_batch),

@@ -1,0 +1,138 @@
+// Drives the C++ simulator surface (include/cudaEM/simulator.hpp) the way
+// user code of the reference does, and writes the results as raw float32
+// files for tests/test_gpu_simulators.py to compare with the oracle.
+//
+//   test_simulators vibrate <out_dir>   phonon displacements only (no GPU)
+//   test_simulators run <out_dir>       HRTEM, ED, CBED, STEM (tile and
+//                                       general scanning), thickness series
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cudaEM/simulator.hpp"
+
+using namespace cudaEM;
+
+static void write(const std::string& path, const std::vector<float>& v) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw simulation_error("cannot write " + path);
+  std::fwrite(v.data(), sizeof(float), v.size(), f);
+  std::fclose(f);
+}
+
+// SrTiO3 perovskite, n x n x nz cells (a = 3.905 A)
+static Atom_Coord_Data<float> srtio3(int n, int nz) {
+  const double a = 3.905;
+  const double basis[5][4] = {{38, 0, 0, 0}, {22, .5, .5, .5}, {8, .5, .5, 0}, {8, .5, 0, .5}, {8, 0, .5, .5}};
+  Atom_Coord_Data<float> at;
+  for (int b = 0; b < 5; ++b)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        for (int k = 0; k < nz; ++k)
+          at.push_back((int)basis[b][0], (float)((i + basis[b][1]) * a), (float)((j + basis[b][2]) * a),
+                       (float)((k + basis[b][3]) * a));
+  at.x_l = at.y_l = (float)(n * a);
+  return at;
+}
+
+static multislice_params<float> base(int n, int cells, int nz) {
+  multislice_params<float> p;
+  p.atoms = srtio3(cells, nz);
+  p.nx = p.ny = n;
+  p.lx = p.ly = cells * 3.905f;
+  p.dz = 3.905f / 2;
+  p.E_0 = 300;
+  p.cond_lens.outer_aper_ang = 20;
+  p.detector.ang_inner = {0, 40};
+  p.detector.ang_outer = {20, 120};
+  return p;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s vibrate|run <out_dir>\n", argv[0]);
+    return 2;
+  }
+  const std::string mode = argv[1], out = argv[2];
+  try {
+    if (mode == "vibrate") {
+      Atom_Coord_Data<double> at, v;
+      for (int i = 0; i < 7; ++i) at.push_back(8, 0.5 * i, 1.0 * i, 0.25 * i);
+      const bool dims[3] = {true, true, true};
+      vibrate_atom_positions(at, v, 1234ull, 3, 0.08, dims);
+      std::vector<float> r;
+      for (size_t i = 0; i < v.size(); ++i) {
+        r.push_back((float)(v.x[i] - at.x[i]));
+        r.push_back((float)(v.y[i] - at.y[i]));
+        r.push_back((float)(v.z[i] - at.z[i]));
+      }
+      write(out + "/vibrate.f32", r);
+      return 0;
+    }
+    mulSimulator sim;
+    {  // HRTEM image (plane wave, Cs = 1 mm, defocus -50 nm)
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_HRTEM;
+      p.obj_lens.c_30 = 1e7f;
+      p.obj_lens.c_10 = -500.f;
+      output_Multi_Data<float> o;
+      sim.executionTEM(p, &o);
+      write(out + "/hrtem.f32", o.image_intensity);
+      std::printf("hrtem %zu\n", o.image_intensity.size());
+    }
+    {  // ED, full wave
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_ED;
+      p.data_record = eWave;
+      output_Multi_Data<float> o;
+      sim.executionTEM(p, &o);
+      write(out + "/ed_re.f32", o.image_real);
+      write(out + "/ed_im.f32", o.image_imag);
+      std::printf("ed %zu\n", o.image_real.size());
+    }
+    {  // CBED at (5, 6), two frozen-phonon configurations
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_CBED;
+      p.incident_wave_x_pos = 5.f;
+      p.incident_wave_y_pos = 6.f;
+      p.phonon_model = ePM_Frozen_Phonon;
+      p.pn_nconf = 2;
+      p.pn_seed = 77;
+      p.pn_sigma = 0.08f;
+      output_Multi_Data<float> o;
+      sim.executionCBED(p, &o);
+      write(out + "/cbed.f32", o.image_intensity);
+      std::printf("cbed %zu\n", o.image_intensity.size());
+    }
+    for (int general = 0; general < 2; ++general) {  // STEM, 4 x 3 scan
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_STEM;
+      p.STEM_col = 4;
+      p.STEM_row = 3;
+      p.probe_batch = 5;
+      p.stem_scanning = general ? eGeneral_scanning : eTile_scanning;
+      output_Multi_Data<float> o;
+      sim.executionSTEM(p, &o);
+      write(out + (general ? "/stem_general.f32" : "/stem_tile.f32"), o.STEM_intensity);
+      write(out + (general ? "/stem_general_total.f32" : "/stem_tile_total.f32"), {o.total_intensity});
+      std::printf("stem %d %zu total %g\n", general, o.STEM_intensity.size(), o.total_intensity);
+    }
+    {  // thickness series through detector 1, every 2 slices
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_Thickness;
+      p.iw_type = eIWT_Convergent_Wave;
+      p.incident_wave_x_pos = 2.f;
+      p.incident_wave_y_pos = 3.f;
+      p.thickness_stride = 2;
+      output_Multi_Data<float> o;
+      sim.executionThickness(p, &o);
+      write(out + "/thickness.f32", o.thickness_frames);
+      std::printf("thickness %zu\n", o.thickness_frames.size());
+    }
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -64,7 +64,8 @@ def test_validation_before_device_work(C):
     assert L.em_plan_create(ctypes.byref(g), ctypes.byref(b), 0, ctypes.byref(h)) == C.EM_ERR_INVALID
     assert L.em_plan_destroy(None) == C.EM_ERR_INVALID
     assert L.em_stem(None, None, 0, None, None, 0, None, None, 1.0, 0, None) == C.EM_ERR_INVALID
-    assert L.em_multislice_forward(None, None, None, 0, None, 0, None) == C.EM_ERR_INVALID
+    assert L.em_multislice_forward(None, None, None, 0, None, 0, None, 1) == C.EM_ERR_INVALID
+    assert L.em_multislice_adjoint(None, None, None, 0, None, 1, None, 0, None, None, None) == C.EM_ERR_INVALID
     assert L.em_multislice_ragged(0, None, None, None, None, None, 0, None, None, 1, None, None) == C.EM_ERR_INVALID
 
 

@@ -232,7 +232,7 @@ def test_adjoint_matches_oracle(em):
         psi0 = _np(psi).astype(np.complex128)
         tt = torch.from_numpy(t.astype(np.complex64)).cuda()
         ck = torch.empty((S, B, n, n), dtype=torch.complex64, device="cuda")
-        p.multislice_forward(psi, t=tt, ckpt=ck)
+        p.multislice_forward(psi, t=tt, ckpt=ck, every=1)
         Psi = _np(psi).astype(np.complex128)
         target = rng.uniform(0.5, 1.5, (B, n, n)) * np.abs(Psi) ** 2
         gPsi = 4 * (np.abs(Psi) ** 2 - target) * Psi
@@ -278,3 +278,64 @@ def test_ragged_matches_separate_runs(em):
         assert rel_l2(_np(g), r) < 1e-5
     for p in plans:
         p.close()
+
+
+def test_adjoint_recompute_and_potential_gradient(em):
+    """Checkpointing every 3rd slice with recompute gives the same gradient as
+    keeping every slice; the potential gradient is Re(conj(gt) i t)."""
+    import torch
+    rng = np.random.default_rng(6)
+    n, S, B, L = 128, 7, 2, 12.0
+    V = rng.uniform(0, 0.8, (S, n, n)).astype(np.float32)
+    with em.Plan(n, n, L, L, 1.5, 300.0, max_batch=B) as p:
+        psi0 = p.build_probes(probe_lens(25.0), [3.0, 8.0], [4.0, 9.0])
+        Vt = torch.from_numpy(V).cuda()
+        res = {}
+        for every in (1, 3):
+            psi = psi0.clone()
+            ck = torch.empty((p.n_checkpoints(S, every), B, n, n), dtype=torch.complex64, device="cuda")
+            p.multislice_forward(psi, V=Vt, ckpt=ck, every=every)
+            g = (psi * 1e-6).contiguous()
+            gV = torch.empty_like(Vt)
+            gt = p.multislice_adjoint(ck, g, V=Vt, gV=gV, every=every)
+            res[every] = (_np(gt), _np(gV), _np(psi))
+        P = _np(p.propagator()).astype(np.complex128)
+        p0 = _np(psi0).astype(np.complex128)
+    assert rel_l2(res[3][0], res[1][0]) < 1e-6
+    assert rel_l2(res[3][1], res[1][1]) < 1e-6
+    t = np.exp(1j * V.astype(np.float64))
+    gt_o, Psi_o = O.adjoint(P, t, p0, res[1][2].astype(np.complex128) * 1e-6)
+    assert rel_l2(res[1][0], gt_o) < TOL_INT
+    gV_o = np.real(np.conj(gt_o) * 1j * t)
+    assert rel_l2(res[1][1], gV_o) < TOL_INT
+
+
+def test_autograd_function_matches_oracle_gradient(em):
+    """Inverse-multislice loss of the differentiable-inversion workload shape
+    (reduced): sum of squared far-field intensity differences; gradient w.r.t.
+    the per-slice transmission through torch.autograd vs the FP64 oracle."""
+    import torch
+    from cudaem_amd.autograd import multislice
+    rng = np.random.default_rng(12)
+    n, S, B, L = 128, 6, 4, 20.0
+    t = (rng.uniform(0.95, 1.0, (S, n, n)) * np.exp(1j * rng.uniform(0, 0.5, (S, n, n))))
+    with em.Plan(n, n, L, L, 2.0, 300.0, max_batch=B) as p:
+        psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, L, B), rng.uniform(0, L, B))
+        tt = torch.from_numpy(t.astype(np.complex64)).cuda().requires_grad_(True)
+        Psi = multislice(p, tt, psi0, every=2)
+        I = Psi.real ** 2 + Psi.imag ** 2
+        scale = 1.0 / float(I.detach().max())
+        target = torch.from_numpy(rng.uniform(0.5, 1.5, (B, n, n)).astype(np.float32)).cuda() * I.detach()
+        loss = (((I - target) * scale) ** 2).sum()
+        loss.backward()
+        g = _np(tt.grad)
+        P = _np(p.propagator()).astype(np.complex128)
+        p0 = _np(psi0).astype(np.complex128)
+        tgt = _np(target).astype(np.float64)
+    Psi_o = np.stack([O.multislice(P, p0[b], t=t) for b in range(B)])
+    I_o = abs(Psi_o) ** 2
+    gPsi = 4 * scale ** 2 * (I_o - tgt) * Psi_o
+    gt_o, _ = O.adjoint(P, t, p0, gPsi)
+    e = rel_l2(g, gt_o)
+    print("autograd gradient relL2", e)
+    assert e < TOL_INT
