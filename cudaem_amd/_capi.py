@@ -34,7 +34,7 @@ EXPORTS = [
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_adjoint",
     "em_detector_sums", "em_ctf_image", "em_diffraction", "em_stem", "em_thickness",
     "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_spline",
-    "em_host_constants",
+    "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
 ]
 
 
@@ -49,6 +49,11 @@ class em_beam(ctypes.Structure):
 
 class em_lens(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in LENS_FIELDS]
+
+
+class em_tuning(ctypes.Structure):
+    _fields_ = [("row_blocks", ctypes.c_int), ("col_blocks", ctypes.c_int), ("col_tile", ctypes.c_int),
+                ("probe_group", ctypes.c_int)]
 
 
 class em_atoms(ctypes.Structure):
@@ -96,6 +101,9 @@ _SIGS = {
     "em_host_slicing": ([ctypes.c_longlong, _PD, _D, _I, _I, _PI, _PD, _PLL, _PLL], _I),
     "em_host_spline": ([_I, _I, _D, _D, _D, _PD, _PD, _PD, _PD, _PD], _I),
     "em_host_constants": ([_D, _PD, _PD, _PD], _I),
+    "em_plan_get_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),
+    "em_plan_set_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),
+    "em_plan_autotune": ([_P, _I, _I, _PD], _I),
 }
 
 _lib = None

@@ -70,7 +70,7 @@ struct em_plan_s {
   Constants k{};
   double vr_factor = 0, gx0 = 0, gy0 = 0;
   GridDev gdev{};
-  int probe_group = 0;
+  em_tuning tune{2048, 512, 0, 0};
   std::string err;
   // spline settings and element overrides
   int nR = 128;
@@ -132,13 +132,14 @@ em_status upload_twiddles(em_plan p, DevBuf& buf, int n) {
 // and as many probes as possible sharing one register-held t or P.
 int row_pb(const em_plan p, int B) {
   const int blocks = p->grid.ny / row_lines_per_block(p->grid.nx);
-  const int target = 2048;
+  const int target = std::max(1, p->tune.row_blocks);
   int groups = std::max(1, std::min(B, (target + blocks - 1) / blocks));
   return (B + groups - 1) / groups;
 }
 int col_pb(const em_plan p, int B) {
-  const int blocks = col_ntiles(p->grid.nx, p->grid.ny);
-  const int target = 512;
+  const int tile = p->tune.col_tile ? p->tune.col_tile : col_tile(p->grid.ny);
+  const int blocks = p->grid.nx / tile;
+  const int target = std::max(1, p->tune.col_blocks);
   int groups = std::max(1, std::min(B, (target + blocks - 1) / blocks));
   return (B + groups - 1) / groups;
 }
@@ -169,6 +170,7 @@ ColArgs col_args(em_plan p, const float2* in, float2* out, int B) {
   a.P = p->P.as<float2>();
   a.tw = p->tw_y.as<float2>();
   a.grid = p->gdev;
+  a.cols = p->tune.col_tile;
   return a;
 }
 
@@ -207,7 +209,7 @@ em_status forward_group(em_plan p, const float* V, const float2* t, int S, float
 }
 
 int group_size(em_plan p, int B) {
-  if (p->probe_group > 0) return std::min(B, p->probe_group);
+  if (p->tune.probe_group > 0) return std::min(B, p->tune.probe_group);
   // keep a group's waves (8 B per pixel) within ~128 MiB so a slice's two
   // passes can be served from the 256 MiB last-level cache
   const long long bytes = p->N() * 8;
@@ -277,7 +279,23 @@ em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double*
 
 em_status em_plan_set_probe_group(em_plan plan, int g) {
   if (!plan || g < 0) return EM_ERR_INVALID;
-  plan->probe_group = g;
+  plan->tune.probe_group = g;
+  plan->graphs.clear();
+  return EM_OK;
+}
+
+em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
+  if (!plan || !t) return EM_ERR_INVALID;
+  *t = plan->tune;
+  return EM_OK;
+}
+
+em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
+  if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0) return EM_ERR_INVALID;
+  if (t->col_tile != 0 && t->col_tile != col_tile(plan->grid.ny) &&
+      (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0))
+    return EM_ERR_INVALID;
+  plan->tune = *t;
   plan->graphs.clear();
   return EM_OK;
 }
@@ -984,4 +1002,75 @@ extern "C" em_status em_host_constants(double e0_kv, double* lambda, double* gam
   if (gamma) *gamma = k.gamma;
   if (sigma) *sigma = k.sigma;
   return EM_OK;
+}
+
+extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double* best) {
+  if (!p || nprobe <= 0 || nslices <= 0) return EM_ERR_INVALID;
+  const long long N = p->N();
+  float* V = nullptr;
+  float2* w = nullptr;
+  EM_HIP(p, hipMalloc(&V, sizeof(float) * N * nslices));
+  if (hipMalloc(&w, sizeof(float2) * N * nprobe) != hipSuccess) {
+    (void)hipFree(V);
+    return p->fail(EM_ERR_ALLOC, "autotune scratch");
+  }
+  hipEvent_t e0, e1;
+  em_status st = p->hip(hipMemsetAsync(V, 0, sizeof(float) * N * nslices, p->stream), "memset");
+  if (st == EM_OK) st = p->hip(hipMemsetAsync(w, 0, sizeof(float2) * N * nprobe, p->stream), "memset");
+  if (st == EM_OK) st = p->hip(hipEventCreate(&e0), "event");
+  if (st == EM_OK) st = p->hip(hipEventCreate(&e1), "event");
+  auto measure = [&](double* ms) -> em_status {
+    EM_TRY(em_multislice_forward(p, V, nullptr, nslices, w, nprobe, nullptr, 1));  // warm
+    EM_HIP(p, hipEventRecord(e0, p->stream));
+    for (int i = 0; i < 2; ++i) EM_TRY(em_multislice_forward(p, V, nullptr, nslices, w, nprobe, nullptr, 1));
+    EM_HIP(p, hipEventRecord(e1, p->stream));
+    EM_HIP(p, hipEventSynchronize(e1));
+    float t = 0;
+    EM_HIP(p, hipEventElapsedTime(&t, e0, e1));
+    *ms = t / 2.0 / ((double)nprobe * nslices);
+    return EM_OK;
+  };
+  double cur = 0;
+  if (st == EM_OK) st = measure(&cur);
+  // coordinate descent over the launch-shape parameters, two sweeps
+  const int groups[] = {0, 4, 8, 32, nprobe};
+  const int rowb[] = {1024, 2048, 4096, 8192};
+  const int colb[] = {256, 512, 1024, 2048};
+  const int tiles[] = {0, col_alt_tile(p->grid.ny)};
+  for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
+    for (int par = 0; par < 4 && st == EM_OK; ++par) {
+      const int* cand = par == 0 ? groups : par == 1 ? rowb : par == 2 ? colb : tiles;
+      const int nc = par == 0 ? 5 : par == 3 ? 2 : 4;
+      em_tuning bestt = p->tune;
+      for (int i = 0; i < nc && st == EM_OK; ++i) {
+        em_tuning t = p->tune;
+        if (par == 0) t.probe_group = cand[i];
+        if (par == 1) t.row_blocks = cand[i];
+        if (par == 2) t.col_blocks = cand[i];
+        if (par == 3) {
+          if (cand[i] != 0 && p->grid.nx % cand[i] != 0) continue;
+          t.col_tile = cand[i];
+        }
+        if (par == 0 && cand[i] > nprobe) continue;
+        const em_tuning keep = p->tune;
+        p->tune = t;
+        double ms = 0;
+        st = measure(&ms);
+        if (st == EM_OK && ms < cur) {
+          cur = ms;
+          bestt = t;
+        }
+        p->tune = keep;
+      }
+      p->tune = bestt;
+    }
+  }
+  p->graphs.clear();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(V);
+  (void)hipFree(w);
+  if (best) *best = cur;
+  return st;
 }

@@ -208,6 +208,69 @@ struct LineLds {
   __device__ __forceinline__ static int idx(int i) { return i + (i >> 4); }
 };
 
+// --------------------------------------------------------------------------
+// Per-thread twiddles of one Stockham stage (R, NS): W_{NS R}^{r k} with
+// k = j mod NS for the butterflies j = tau + T u of this thread.
+//  mode 1 (T multiple of NS):  k = tau mod NS for every u;
+//  mode 2 (NS multiple of T):  k = tau + T (u mod NS/T), and the u part
+//                              W^{r T u} is a compile-time constant;
+//  mode 0 (otherwise):         read from the table at every use.
+// For modes 1 and 2 the factors are W^{r k} = W^{r1 k} W^{4 r2 k}
+// (r = r1 + 4 r2, r1 < 4): 3 + (R-1)/4 table values per thread, read once
+// from tw[m] = exp(-2 pi i m / N) (FP64-generated) and kept in registers.
+// --------------------------------------------------------------------------
+template <int N, int T, int NS, int R>
+struct StageTw {
+  static constexpr int NR = NS * R;
+  static constexpr int step = N / NR;
+  static constexpr int mode = (NS <= 1) ? 3 : ((T % NS == 0) ? 1 : ((NS % T == 0) ? 2 : 0));
+  static constexpr int NLO = (R - 1 < 3) ? (R - 1) : 3;
+  static constexpr int NHI = (R - 1) / 4;
+  float2 lo[NLO > 0 ? NLO : 1];
+  float2 hi[NHI > 0 ? NHI : 1];
+  int k0;
+
+  __device__ __forceinline__ void init(int tau, const float2* __restrict__ tw) {
+    if constexpr (mode == 1 || mode == 2) {
+      const int k = (mode == 1) ? (tau % NS) : tau;
+#pragma unroll
+      for (int m = 1; m <= NLO; ++m) lo[m - 1] = tw[(m * k * step) % N];
+#pragma unroll
+      for (int m = 1; m <= NHI; ++m) hi[m - 1] = tw[(4 * m * k * step) % N];
+    }
+    k0 = tau;
+  }
+
+  // v *= W^{r k} for input r of butterfly u (compile-time r, u)
+  template <int DIR, int U, int RR>
+  __device__ __forceinline__ float2 apply(float2 v, const float2* __restrict__ tw) const {
+    if constexpr (mode == 0) {
+      const int k = (k0 + T * U) % NS;
+      return twmul<DIR>(v, tw[(RR * k * step) % N]);
+    } else {
+      constexpr int r1 = RR % 4, r2 = RR / 4;
+      if constexpr (r1 > 0) v = twmul<DIR>(v, lo[r1 - 1]);
+      if constexpr (r2 > 0) v = twmul<DIR>(v, hi[r2 - 1]);
+      if constexpr (mode == 2) v = ctw<NR, RR * T * (U % (NS / T)), DIR>(v);
+      return v;
+    }
+  }
+};
+
+template <int N, int T, int NS, int... RS>
+struct TwChain {
+  __device__ __forceinline__ void init(int, const float2*) {}
+};
+template <int N, int T, int NS, int R, int... RS>
+struct TwChain<N, T, NS, R, RS...> {
+  StageTw<N, T, NS, R> s;
+  TwChain<N, T, NS * R, RS...> next;
+  __device__ __forceinline__ void init(int tau, const float2* __restrict__ tw) {
+    s.init(tau, tw);
+    next.init(tau, tw);
+  }
+};
+
 // Synchronisation between the LDS write and read phases of an exchange.
 // SYNC_BLOCK: lines span several wavefronts (workgroup barrier).
 // SYNC_WAVE:  every line lives inside one wavefront; LDS requests of one wave
@@ -225,10 +288,9 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 // --------------------------------------------------------------------------
-// Stockham stages.  Stage (R, Ns): butterfly j in [0, N/R), thread tau owns
+// Stockham stages.  Stage (R, NS): butterfly j in [0, N/R), thread tau owns
 // j = tau + T*u, u in [0, E/R); its inputs x[j + r N/R] sit in v[u + r E/R];
-// twiddle W_{Ns R}^{r (j mod Ns)}; outputs go to (j/Ns) Ns R + (j mod Ns) + r Ns.
-// Twiddles come from tw[m] = exp(-2 pi i m / N) (FP64-generated table).
+// twiddle W_{NS R}^{r (j mod NS)}; outputs go to (j/NS) NS R + (j mod NS) + r NS.
 // --------------------------------------------------------------------------
 template <int N, int T, int DIR, int SYNC>
 struct FftLine {
@@ -242,19 +304,15 @@ struct FftLine {
     }
   }
 
-  template <int R, int NS>
-  __device__ __forceinline__ static void twiddle(float2 (&v)[E], int tau, const float2* __restrict__ tw) {
-    if constexpr (NS > 1) {
-      constexpr int step = N / (NS * R);
-#pragma unroll
-      for (int u = 0; u < E / R; ++u) {
-        const int j = tau + T * u;
-        const int k = j % NS;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const float2 w = tw[(r * k * step) % N];
-          v[u + r * (E / R)] = twmul<DIR>(v[u + r * (E / R)], w);
-        }
+  template <int R, int NS, int U, int RR>
+  __device__ __forceinline__ static void twiddle_one(float2 (&v)[E], const StageTw<N, T, NS, R>& st,
+                                                     const float2* __restrict__ tw) {
+    if constexpr (U < E / R) {
+      if constexpr (RR < R) {
+        v[U + RR * (E / R)] = st.template apply<DIR, U, RR>(v[U + RR * (E / R)], tw);
+        twiddle_one<R, NS, U, RR + 1>(v, st, tw);
+      } else {
+        twiddle_one<R, NS, U + 1, 1>(v, st, tw);
       }
     }
   }
@@ -278,16 +336,17 @@ struct FftLine {
 
   template <int NS, int R, int... RS>
   __device__ __forceinline__ static void stages(float2 (&v)[E], float2* lds, int tau,
+                                                const TwChain<N, T, NS, R, RS...>& tws,
                                                 const float2* __restrict__ tw) {
     static_assert(E % R == 0, "radix must divide the elements per thread");
-    twiddle<R, NS>(v, tau, tw);
+    if constexpr (NS > 1) twiddle_one<R, NS, 0, 1>(v, tws.s, tw);
     butterflies<R, 0>(v);
     if constexpr (sizeof...(RS) > 0) {
       store_stage<R, NS>(v, lds, tau);
       lds_sync<SYNC>();
       load_strided(v, lds, tau);
       lds_sync<SYNC>();
-      stages<NS * R, RS...>(v, lds, tau, tw);
+      stages<NS * R, RS...>(v, lds, tau, tws.next, tw);
     } else {
       static_assert(NS * R == N, "radices must multiply to N");
     }
@@ -295,8 +354,9 @@ struct FftLine {
 
   template <int... RS>
   __device__ __forceinline__ static void run(float2 (&v)[E], float2* lds, int tau,
+                                             const TwChain<N, T, 1, RS...>& tws,
                                              const float2* __restrict__ tw) {
-    stages<1, RS...>(v, lds, tau, tw);
+    stages<1, RS...>(v, lds, tau, tws, tw);
   }
 };
 

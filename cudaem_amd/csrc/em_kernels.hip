@@ -22,10 +22,12 @@ struct FftCfg;
   template <>                                                                       \
   struct FftCfg<N_> {                                                               \
     static constexpr int T = T_;                                                    \
+    using Tw = TwChain<N_, T_, 1, __VA_ARGS__>;                                     \
     template <int DIR, int SYNC>                                                    \
     __device__ __forceinline__ static void run(float2 (&v)[N_ / T_], float2* lds,   \
-                                               int tau, const float2* tw) {         \
-      FftLine<N_, T_, DIR, SYNC>::template run<__VA_ARGS__>(v, lds, tau, tw);       \
+                                               int tau, const Tw& tws,              \
+                                               const float2* tw) {                  \
+      FftLine<N_, T_, DIR, SYNC>::template run<__VA_ARGS__>(v, lds, tau, tws, tw);  \
     }                                                                               \
   };
 EM_FFT_CFG(64, 8, 8, 8)
@@ -39,11 +41,20 @@ EM_FFT_CFG(2048, 128, 16, 16, 8)
 
 constexpr int kRowThreads = 256;
 
-// Columns per column-pass workgroup for a column length N.
+// Columns per column-pass workgroup: the default for a column length N, and
+// the alternative the autotuner may pick for the per-slice modes.
 template <int N>
+struct DefaultCols {
+  static constexpr int value = (FftCfg<N>::T >= 128) ? 8 : 16;
+};
+template <int N>
+struct AltCols {
+  static constexpr int value = DefaultCols<N>::value / 2;
+};
+template <int N, int COLS_ = DefaultCols<N>::value>
 struct ColCfg {
   static constexpr int T = FftCfg<N>::T;
-  static constexpr int COLS = (T >= 128) ? 8 : 16;
+  static constexpr int COLS = COLS_;
   static constexpr int THREADS = COLS * T;
   static constexpr int STRIDE = LineLds<N>::size + 1;  // odd stride: columns on distinct banks
   static constexpr size_t LDS = (size_t)COLS * STRIDE * sizeof(float2);
@@ -65,6 +76,9 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
   const int b1 = min(a.nprobe, b0 + a.pb);
   float2* lds = smem + line * LineLds<NX>::size;
   const long long row = (long long)y * NX;
+  constexpr int SYNC = (T <= 64) ? SYNC_WAVE : SYNC_BLOCK;  // a line inside one wavefront
+  typename FftCfg<NX>::Tw tws;
+  tws.init(tau, a.tw);
 
   // transmission values of this thread's elements, reused for every probe
   float2 t[E];
@@ -98,7 +112,7 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = a.in[base + tau + T * e];
     }
-    if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) a.ckpt[base + tau + T * e] = v[e];
@@ -116,7 +130,7 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = cmul(v[e], t[e]);
     }
-    if constexpr ((MODE & ROW_POST_FFT) != 0) FftCfg<NX>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & ROW_POST_FFT) != 0) FftCfg<NX>::template run<-1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_OUT_ABS2) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -159,9 +173,9 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
   return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
 }
 
-template <int NY, unsigned MODE>
-__global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a) {
-  using C = ColCfg<NY>;
+template <int NY, int COLS_, unsigned MODE>
+__global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) void col_pass_kernel(ColArgs a) {
+  using C = ColCfg<NY, COLS_>;
   constexpr int T = C::T;
   constexpr int E = NY / T;
   constexpr int COLS = C::COLS;
@@ -173,6 +187,8 @@ __global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a
   const int b0 = blockIdx.y * a.pb;
   const int b1 = min(a.nprobe, b0 + a.pb);
   float2* lds = smem + c * C::STRIDE;
+  typename FftCfg<NY>::Tw tws;
+  tws.init(tau, a.tw);
 
   float2 p[(MODE & (COL_MUL_P | COL_CONJ_P | COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0 ? E : 1];
   if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
@@ -222,8 +238,8 @@ __global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a
         for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
       }
     }
-    if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
-    if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
@@ -253,8 +269,8 @@ __global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a
       }
       __syncthreads();
     }
-    if constexpr ((MODE & COL_POST_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, a.tw);
-    if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, a.tw);
+    if constexpr ((MODE & COL_POST_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & COL_STORE) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) a.out[base + (long long)(tau + T * e) * nx] = v[e];
@@ -303,6 +319,11 @@ __global__ __launch_bounds__(ColCfg<NY>::THREADS) void col_pass_kernel(ColArgs a
   X(COL_PRE_IFFT | COL_CONJ_P | COL_POST_FFT | COL_STORE)                       \
   X(COL_PRE_IFFT | COL_MUL_P | COL_POST_FFT | COL_STORE)
 
+// per-slice column modes that also exist with the alternative column tile
+#define EM_COL_HOT_MODES(X)                                                     \
+  X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
+  X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)
+
 #define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
 bool fft_size_supported(int n) {
@@ -333,11 +354,11 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   }
 }
 
-template <int NY, unsigned M>
+template <int NY, int COLS, unsigned M>
 static hipError_t col_launch_one(const ColArgs& a, hipStream_t s) {
-  using C = ColCfg<NY>;
+  using C = ColCfg<NY, COLS>;
   const dim3 grid(a.nx / C::COLS, (a.nprobe + a.pb - 1) / a.pb);
-  hipLaunchKernelGGL((col_pass_kernel<NY, M>), grid, dim3(C::THREADS), C::LDS, s, a);
+  hipLaunchKernelGGL((col_pass_kernel<NY, COLS, M>), grid, dim3(C::THREADS), C::LDS, s, a);
   return hipGetLastError();
 }
 
@@ -346,10 +367,18 @@ static hipError_t col_attrs(void) {
   hipError_t e = hipSuccess;
 #define EM_COL_ATTR(M)                                                                   \
   if (e == hipSuccess)                                                                   \
-    e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, (M)>,                        \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ColCfg<NY>::LDS);
+    e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, (M)>, \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                  \
+                            (int)ColCfg<NY>::LDS);
   EM_COL_MODES(EM_COL_ATTR)
 #undef EM_COL_ATTR
+#define EM_COL_ATTR_ALT(M)                                                               \
+  if (e == hipSuccess)                                                                   \
+    e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, AltCols<NY>::value, (M)>,   \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                  \
+                            (int)ColCfg<NY, AltCols<NY>::value>::LDS);
+  EM_COL_HOT_MODES(EM_COL_ATTR_ALT)
+#undef EM_COL_ATTR_ALT
   return e;
 }
 
@@ -367,10 +396,21 @@ hipError_t prepare_col_kernels(int ny) {
 
 template <int NY>
 static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
+  if (a.cols == AltCols<NY>::value) {
+    switch (mode) {
+#define EM_COL_ALT_CASE(M) \
+  case (M):                \
+    return col_launch_one<NY, AltCols<NY>::value, (M)>(a, s);
+      EM_COL_HOT_MODES(EM_COL_ALT_CASE)
+#undef EM_COL_ALT_CASE
+      default:
+        break;
+    }
+  }
   switch (mode) {
 #define EM_COL_CASE(M) \
   case (M):            \
-    return col_launch_one<NY, (M)>(a, s);
+    return col_launch_one<NY, DefaultCols<NY>::value, (M)>(a, s);
     EM_COL_MODES(EM_COL_CASE)
 #undef EM_COL_CASE
     default:
@@ -393,6 +433,8 @@ hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s) {
 
 hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s) {
   if (a.nx % col_tile(a.ny) != 0 || a.pb <= 0 || a.ndet > kMaxDet) return hipErrorInvalidValue;
+  if (a.cols != 0 && a.cols != col_tile(a.ny) && (a.cols != col_alt_tile(a.ny) || a.nx % a.cols != 0))
+    return hipErrorInvalidValue;
   switch (a.ny) {
 #define EM_CASE(N) \
   case N:          \
@@ -421,6 +463,18 @@ int col_tile(int ny) {
 #define EM_CASE(N) \
   case N:          \
     return ColCfg<N>::COLS;
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return 1;
+  }
+}
+
+int col_alt_tile(int ny) {
+  switch (ny) {
+#define EM_CASE(N) \
+  case N:          \
+    return AltCols<N>::value;
     EM_SIZES(EM_CASE)
 #undef EM_CASE
     default:

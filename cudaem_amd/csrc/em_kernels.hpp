@@ -86,6 +86,7 @@ struct ColArgs {
   int ndet;
   double dg2min[kMaxDet], dg2max[kMaxDet];
   float* dpart;             // [nprobe][ndet][ntiles]
+  int cols;                 // column tile of the per-slice modes: 0 = default, or col_alt_tile
 };
 
 hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
@@ -95,7 +96,8 @@ hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
 hipError_t prepare_col_kernels(int ny);
 bool fft_size_supported(int n);
 int row_lines_per_block(int nx);
-int col_tile(int ny);
+int col_tile(int ny);      // default columns per column-pass workgroup
+int col_alt_tile(int ny);  // the alternative tile of the per-slice column modes
 int col_ntiles(int nx, int ny);
 
 // --- other kernels ------------------------------------------------------------

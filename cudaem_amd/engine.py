@@ -104,6 +104,24 @@ class Plan:
     def set_probe_group(self, g):
         C.check(C.lib().em_plan_set_probe_group(self._h, int(g)), self._h)
 
+    def tuning(self):
+        t = C.em_tuning()
+        C.check(C.lib().em_plan_get_tuning(self._h, ctypes.byref(t)), self._h)
+        return {k: getattr(t, k) for k, _ in C.em_tuning._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = C.em_tuning(**cur)
+        C.check(C.lib().em_plan_set_tuning(self._h, ctypes.byref(t)), self._h, "set_tuning")
+
+    def autotune(self, nprobe, nslices=4):
+        """Times candidate launch shapes on this grid and keeps the fastest;
+        returns (tuning, ms per probe-slice)."""
+        ms = ctypes.c_double()
+        self._call("em_plan_autotune", int(nprobe), int(nslices), ctypes.byref(ms))
+        return self.tuning(), ms.value
+
     def set_element_coefs(self, Z, a, b, c=(), d=()):
         a, b, c, d = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, d))
         C.check(C.lib().em_set_element_coefs(self._h, int(Z), len(a), _dptr(a), _dptr(b), len(c),

@@ -86,6 +86,23 @@ em_status em_plan_set_stream(em_plan plan, void* hip_stream);
 em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double* gamma);
 /* Group of probes a fused pass keeps together (0 = automatic). */
 em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
+/* Launch shape of the fused passes (no reference counterpart: the reference
+ * relies on cuFFT's planner).  row_blocks / col_blocks: target workgroups of a
+ * row / column pass (the probes each workgroup processes follow from it);
+ * col_tile: columns per column-pass workgroup (0 = default, or the
+ * alternative tile of the grid size); probe_group: probes carried together
+ * through all slices (0 = as many as fit 128 MiB, the working set the
+ * last-level cache can hold). */
+typedef struct em_tuning {
+  int row_blocks, col_blocks, col_tile, probe_group;
+} em_tuning;
+em_status em_plan_get_tuning(em_plan plan, em_tuning* tuning);
+em_status em_plan_set_tuning(em_plan plan, const em_tuning* tuning);
+/* Autotuner: times candidate tunings (coordinate descent, two sweeps) by
+ * running nslices slices of nprobe synthetic probes on the plan's grid with
+ * HIP events and keeps the fastest; best_ms_per_probe_slice receives its
+ * time.  Allocates and frees its own scratch. */
+em_status em_plan_autotune(em_plan plan, int nprobe, int nslices, double* best_ms_per_probe_slice);
 /* Last error message of this plan (static storage, valid until the next call). */
 const char* em_plan_last_error(em_plan plan);
 int em_fft_size_supported(int n);
