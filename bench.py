@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--group", type=int, default=0, help="probes kept together per fused pass (0 = auto)")
     ap.add_argument("--cpu-probes", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--npos", type=int, default=0, help="scan positions per step (0 = the full 64x64 raster)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--plain-probes", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device 5 times (known bytes for HBM counter scaling)")
     return ap.parse_args()
 
 
@@ -81,6 +86,90 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
     return n_probe * S / dt, used, rel, n_probe * S
 
 
+def plain_path(em, torch, at, xs, ys, n_probe):
+    """The reference's one-probe-per-launch STEM path (run_STEM_general_simu,
+    scan_simulator.cuh:114-183) on this build's kernels: per probe, re-cut
+    the atoms around the probe (2 A margin), slice them, rebuild every
+    potential slice, build the probe, run the slices, sum the detector and
+    copy the result to the host.  Returns probe-slices/s."""
+    L = at["lx"]
+    plan = em.Plan(N_GRID, N_GRID, L, L, DZ, E0, max_batch=1)
+    lens = {"outer_aper_mrad": ALPHA}
+    done = 0
+
+    def one(i):
+        c = em.specimen.cut_window(at, xs[i] - L / 2, xs[i] + L / 2, ys[i] - L / 2, ys[i] + L / 2, margin=2.0)
+        S = plan.set_specimen(c["x"], c["y"], c["z"], c["Z"], c["occ"], border=em.BORDER_MIN)
+        zs, ze, _, _ = plan.slice_info(0)
+        V = plan.potential_slices()
+        psi = plan.build_probes(lens, [0.0], [0.0], defocus_shift=0.5 * (zs + ze))
+        plan.multislice_forward(psi, V=V)
+        plan.detector_sums(psi, DET[0], DET[1]).cpu()
+        return S
+
+    one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_probe):
+        done += one(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.close()
+    return done / dt
+
+
+def adjoint_ratio(em, torch, n=512, S=40, scan=16, reps=3):
+    """Differentiable inverse multislice workload: 512^2, 40 slices of smooth
+    random transmission (phase in [0, 0.5] rad, amplitude 0.95-1), 16 x 16
+    probes at 300 kV / 20 mrad, loss = sum of squared far-field intensity
+    differences to a seeded target.  Returns (forward s, forward+adjoint s)."""
+    g = torch.Generator(device="cuda").manual_seed(5)
+    L = 30.0
+    k = torch.fft.fftfreq(n, device="cuda")
+    lp = torch.exp(-(k[None, :] ** 2 + k[:, None] ** 2) / 2e-3)
+
+    def smooth(lo, hi):
+        f = torch.randn((S, n, n), generator=g, device="cuda")
+        f = torch.fft.ifft2(torch.fft.fft2(f) * lp).real
+        f = (f - f.amin()) / (f.amax() - f.amin())
+        return lo + (hi - lo) * f
+
+    t = (smooth(0.95, 1.0) * torch.exp(1j * smooth(0.0, 0.5))).to(torch.complex64).contiguous()
+    plan = em.Plan(n, n, L, L, 2.0, 300.0, max_batch=scan * scan)
+    xs, ys = em.specimen.raster(scan, scan, L, L)
+    psi0 = plan.build_probes({"outer_aper_mrad": 20.0}, xs, ys)
+    B = psi0.shape[0]
+    target = torch.rand((B, n, n), generator=g, device="cuda")
+    ck = torch.empty((S, B, n, n), dtype=torch.complex64, device="cuda")
+
+    def fwd():
+        psi = psi0.clone()
+        plan.multislice_forward(psi, t=t)
+        I = psi.real ** 2 + psi.imag ** 2
+        return ((I - target) ** 2).sum()
+
+    def fwd_adj():
+        psi = psi0.clone()
+        plan.multislice_forward(psi, t=t, ckpt=ck)
+        I = psi.real ** 2 + psi.imag ** 2
+        gPsi = (4 * (I - target) * psi).contiguous()
+        gt = plan.multislice_adjoint(ck, gPsi, t=t)
+        return gt
+
+    res = []
+    for f in (fwd, fwd_adj):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        res.append((time.perf_counter() - t0) / reps)
+    plan.close()
+    del ck
+    return res[0], res[1]
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -91,12 +180,28 @@ def main():
     import cudaem_amd as em
 
     torch.cuda.init()
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
+                          "launches": 5}))
+        return
     at0 = em.specimen.srtio3(*CELLS, a=A_SRTIO3)
     L = at0["lx"]
     xs, ys = em.specimen.raster(SCAN, SCAN, L, L)
+    if args.npos:
+        xs, ys = xs[:args.npos], ys[:args.npos]
     npos = len(xs)
     plan = em.Plan(N_GRID, N_GRID, L, L, DZ, E0, max_batch=args.batch)
     plan.set_probe_group(args.group)
+    tune_ms = None
+    if not args.no_autotune:
+        tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=4)
+        if args.group:
+            plan.set_tuning(probe_group=args.group)
     lens = {"outer_aper_mrad": ALPHA}
     out = torch.zeros((1, npos), dtype=torch.float32, device="cuda")
 
@@ -123,7 +228,8 @@ def main():
 
     # dominant kernels at the real shape: one group of probes, one slice
     N = N_GRID * N_GRID
-    G = min(args.batch, args.group if args.group > 0 else max(1, (128 << 20) // (N * 8)))
+    tun = plan.tuning()
+    G = min(args.batch, tun["probe_group"] if tun["probe_group"] > 0 else max(1, (128 << 20) // (N * 8)))
     V1 = plan.potential_slices(S // 2, 1)[0].contiguous()
     wave = torch.zeros((G, N_GRID, N_GRID), dtype=torch.complex64, device="cuda")
     wave[:, 0, 0] = 1.0
@@ -132,6 +238,9 @@ def main():
     col_bytes = 16 * N * G + 8 * N
     dom = ("column pass", col_ms, col_bytes) if col_ms >= row_ms else ("row pass", row_ms, row_bytes)
     achieved = dom[2] / (dom[1] * 1e-3)
+
+    plain_ps = plain_path(em, torch, em.specimen.vibrate(at0, U_RMS, SEED, 1), xs, ys, args.plain_probes)
+    t_fwd, t_fwd_adj = adjoint_ratio(em, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -158,12 +267,19 @@ def main():
         "config": {"workload": "batched STEM, SrTiO3 10x10x40 cells, 1024^2 grid, 80 slices, 300 kV, "
                                "25 mrad, 64x64 scan, 1 frozen-phonon configuration per step",
                    "grid": N_GRID, "slices": S, "probes_per_step": npos, "batch": args.batch,
-                   "probe_group": G, "phonon_u_rms": U_RMS, "parallelism": "single GPU"},
+                   "probe_group": G, "tuning": tun, "autotune_ms_per_probe_slice": tune_ms,
+                   "phonon_u_rms": U_RMS, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "launch_ms": dom[1], "algorithmic_bytes_per_launch": dom[2],
                      "row_pass_ms": row_ms, "col_pass_ms": col_ms},
         "slice_roofline_frac": ps * (16 * N + 12 * N / G) / HBM_PEAK,
+        "batched_vs_plain": {"ratio": ps / plain_ps, "plain_probe_slices_per_s": plain_ps,
+                             "plain_path": "one probe per launch, atoms re-cut and potentials rebuilt per probe "
+                                           f"(general scanner), {args.plain_probes} probes"},
+        "adjoint_vs_forward": {"ratio": t_fwd_adj / t_fwd, "forward_s": t_fwd, "forward_plus_adjoint_s": t_fwd_adj,
+                               "workload": "512^2, 40 slices, 16x16 probes, 300 kV / 20 mrad, "
+                                           "checkpoint every slice"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
