@@ -87,7 +87,7 @@ struct em_plan_s {
   float bin = 0;
   // device
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
-  DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, detout, seg, segwave;
+  DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   int vstore_slices = -1;
   std::map<GraphKey, std::unique_ptr<Graph>> graphs;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
@@ -682,6 +682,10 @@ em_status ensure_specimen_potential(em_plan p) {
   EM_HIP(p, p->Vstore.ensure(sizeof(float) * p->N() * S));
   p->vstore_slices = -1;
   EM_TRY(em_potential_slices(p, 0, S, p->Vstore.as<float>()));
+  // transmissions of the stored slices: one sincos per pixel and slice, not
+  // one per pixel, slice and workgroup of the row passes
+  EM_HIP(p, p->Tstore.ensure(sizeof(float2) * p->N() * S));
+  EM_HIP(p, launch_transmission(p->Vstore.as<float>(), p->Tstore.as<float2>(), p->N() * S, p->stream));
   p->vstore_slices = S;
   return EM_OK;
 }
@@ -705,8 +709,8 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
     for (int s = 0; s < S; ++s) {
       RowArgs r = row_args(p, wave, wave, ng);
-      r.V = p->Vstore.as<float>() + (long long)s * N;
-      EM_TRY(row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r));
+      r.t = p->Tstore.as<float2>() + (long long)s * N;
+      EM_TRY(row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r));
       ColArgs cc = col_args(p, wave, wave, ng);
       if (s < S - 1) {
         EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc));
@@ -937,15 +941,18 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
   float2* w = reinterpret_cast<float2*>(wave);
   hipEvent_t e[3];
   for (auto& x : e) EM_HIP(p, hipEventCreate(&x));
+  // the row pass as em_stem runs it: transmission precomputed per slice
+  EM_HIP(p, p->tmp.ensure(sizeof(float2) * p->N()));
+  EM_HIP(p, launch_transmission(V, p->tmp.as<float2>(), p->N(), p->stream));
   RowArgs r = row_args(p, w, w, nprobe);
-  r.V = V;
+  r.t = p->tmp.as<float2>();
   const ColArgs c = col_args(p, w, w, nprobe);
   em_status st = EM_OK;
   // warm up both kernels once
-  st = row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r);
+  st = row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r);
   if (st == EM_OK) st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
   if (st == EM_OK) st = p->hip(hipEventRecord(e[0], p->stream), "event");
-  for (int i = 0; i < iters && st == EM_OK; ++i) st = row(p, ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT, r);
+  for (int i = 0; i < iters && st == EM_OK; ++i) st = row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r);
   if (st == EM_OK) st = p->hip(hipEventRecord(e[1], p->stream), "event");
   for (int i = 0; i < iters && st == EM_OK; ++i)
     st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);

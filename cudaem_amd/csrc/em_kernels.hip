@@ -769,6 +769,22 @@ hipError_t launch_split(const float2* in, float* re, float* im, long long n, flo
   return hipGetLastError();
 }
 
+// t = exp(i V) of sigma-scaled potential slices (generate_trans_function,
+// device_kernels.cuh:106-119)
+__global__ void transmission_kernel(const float* V, float2* t, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s, c;
+    sincosf(V[i], &s, &c);
+    t[i] = make_float2(c, s);
+  }
+}
+
+hipError_t launch_transmission(const float* V, float2* t, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(transmission_kernel, dim3(4096), dim3(256), 0, s, V, t, n);
+  return hipGetLastError();
+}
+
 // dL/dV for t = exp(i V): Re(conj(gt) * i t) * sigma
 __global__ void grad_v_kernel(const float2* gt, const float* V, float sigma, long long n, float* gV) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;

@@ -122,6 +122,7 @@ hipError_t launch_abs2(const float2* in, float* out, long long n, float w, int a
                        hipStream_t s);
 hipError_t launch_split(const float2* in, float* re, float* im, long long n, float w,
                         int accumulate, hipStream_t s);
+hipError_t launch_transmission(const float* V, float2* t, long long n, hipStream_t s);
 hipError_t launch_grad_v(const float2* gt, const float* V, float sigma, long long n, float* gV,
                          hipStream_t s);
 

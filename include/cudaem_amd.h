@@ -226,7 +226,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V_d
 
 /* ---- measurement -----------------------------------------------------------
  * No reference counterpart (the reference has no timing code).  Average
- * device time in ms of the fused row pass (IFFT_x, exp(iV), FFT_x) and column
+ * device time in ms of the fused row pass (IFFT_x, t = exp(iV), FFT_x) and column
  * pass (FFT_y, P, IFFT_y) over nprobe probes at wave_dev[B][N], each launched
  * iters times between HIP events on the plan's stream.  V_dev: one slice
  * [ny][nx].  The wave is overwritten. */

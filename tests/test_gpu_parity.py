@@ -339,3 +339,31 @@ def test_autograd_function_matches_oracle_gradient(em):
     e = rel_l2(g, gt_o)
     print("autograd gradient relL2", e)
     assert e < TOL_INT
+
+
+def test_tuning_variants_agree(em):
+    """Launch shapes (column tile, probes per workgroup, probe group) change
+    scheduling only: every tuning gives the same exit waves, and the autotuner
+    leaves a valid tuning behind."""
+    import torch
+    rng = np.random.default_rng(13)
+    n, S, B = 256, 5, 6
+    V = torch.from_numpy(rng.uniform(0, 1, (S, n, n)).astype(np.float32)).cuda()
+    with em.Plan(n, n, 20.0, 20.0, 2.0, 300.0, max_batch=B) as p:
+        psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, 20, B), rng.uniform(0, 20, B))
+        outs = []
+        alt = 8
+        for tun in (dict(row_blocks=2048, col_blocks=512, col_tile=0, probe_group=0),
+                    dict(row_blocks=64, col_blocks=16, col_tile=alt, probe_group=2),
+                    dict(row_blocks=8192, col_blocks=4096, col_tile=alt, probe_group=5)):
+            p.set_tuning(**tun)
+            psi = psi0.clone()
+            p.multislice_forward(psi, V=V)
+            outs.append(_np(psi))
+        best, ms = p.autotune(B, nslices=2)
+        assert ms > 0 and best["row_blocks"] > 0
+        psi = psi0.clone()
+        p.multislice_forward(psi, V=V)
+        outs.append(_np(psi))
+    for o in outs[1:]:
+        assert rel_l2(o, outs[0]) < 1e-6
