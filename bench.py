@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--npos", type=int, default=0, help="scan positions per step (0 = the full 64x64 raster)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--plain-probes", type=int, default=4)
+    ap.add_argument("--only-stem", action="store_true",
+                    help="time the batched STEM steps only (profiling runs): no pass timing, plain path, "
+                         "adjoint or CPU baseline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (known bytes for HBM counter scaling)")
     return ap.parse_args()
@@ -225,6 +228,11 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ps = args.steps * npos * S / dt
+
+    if args.only_stem:
+        print(json.dumps({"metric": "probe-slices/s (batched STEM steps only)", "value": ps, "steps": args.steps,
+                          "probes_per_step": npos, "tuning": plan.tuning()}), flush=True)
+        return
 
     # dominant kernels at the real shape: one group of probes, one slice
     N = N_GRID * N_GRID

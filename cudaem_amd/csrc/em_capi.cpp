@@ -70,7 +70,7 @@ struct em_plan_s {
   Constants k{};
   double vr_factor = 0, gx0 = 0, gy0 = 0;
   GridDev gdev{};
-  em_tuning tune{2048, 512, 0, 0};
+  em_tuning tune{2048, 512, 0, 0, 0};
   std::string err;
   // spline settings and element overrides
   int nR = 128;
@@ -171,6 +171,8 @@ ColArgs col_args(em_plan p, const float2* in, float2* out, int B) {
   a.tw = p->tw_y.as<float2>();
   a.grid = p->gdev;
   a.cols = p->tune.col_tile;
+  const bool tr = col_transposed_available(p->grid.ny) && (a.cols == 0 || a.cols == 16 || a.cols == 8);
+  a.variant = (p->tune.col_variant == 1 || !tr) ? COL_VARIANT_BLOCK : COL_VARIANT_TRANSPOSED;
   return a;
 }
 
@@ -291,9 +293,12 @@ em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
 }
 
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
-  if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0) return EM_ERR_INVALID;
+  if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0 || t->col_variant < 0 ||
+      t->col_variant > 2)
+    return EM_ERR_INVALID;
   if (t->col_tile != 0 && t->col_tile != col_tile(plan->grid.ny) &&
-      (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0))
+      (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0) &&
+      !(t->col_tile == 8 && col_transposed_available(plan->grid.ny) && plan->grid.nx % 8 == 0))
     return EM_ERR_INVALID;
   plan->tune = *t;
   plan->graphs.clear();
@@ -829,7 +834,10 @@ em_status em_thickness(em_plan p, const float* V, int S, void* psi, int nprobe, 
     ColArgs c = col_args(p, wv, p->tmp.as<float2>(), nprobe);
     c.fg2min = mrad_to_g2(inner, p->k.lambda, false);
     c.fg2max = mrad_to_g2(outer, p->k.lambda, true);
-    c.fscale = (float)(1.0 / (double)N);
+    // the wave here is in the (y, kx) layout, F_x psi / nx (the unnormalised
+    // inverse of the next row pass restores psi), so the band-pass scale 1/N
+    // of Filter_wave becomes nx/N = 1/ny
+    c.fscale = (float)(1.0 / (double)p->grid.ny);
     EM_TRY(col(p, COL_PRE_FFT | COL_MUL_FILTER | COL_POST_IFFT | COL_STORE, c));
     RowArgs ro = row_args(p, p->tmp.as<float2>(), nullptr, nprobe);
     ro.o1 = frames + (long long)nf * nprobe * N;
@@ -1044,10 +1052,11 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   const int rowb[] = {1024, 2048, 4096, 8192};
   const int colb[] = {256, 512, 1024, 2048};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
+  const int variants[] = {1, 2};
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
-    for (int par = 0; par < 4 && st == EM_OK; ++par) {
-      const int* cand = par == 0 ? groups : par == 1 ? rowb : par == 2 ? colb : tiles;
-      const int nc = par == 0 ? 5 : par == 3 ? 2 : 4;
+    for (int par = 0; par < 5 && st == EM_OK; ++par) {
+      const int* cand = par == 0 ? groups : par == 1 ? rowb : par == 2 ? colb : par == 3 ? tiles : variants;
+      const int nc = par == 0 ? 5 : (par == 3 || par == 4) ? 2 : 4;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;
@@ -1058,6 +1067,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
           if (cand[i] != 0 && p->grid.nx % cand[i] != 0) continue;
           t.col_tile = cand[i];
         }
+        if (par == 4) t.col_variant = cand[i];
         if (par == 0 && cand[i] > nprobe) continue;
         const em_tuning keep = p->tune;
         p->tune = t;

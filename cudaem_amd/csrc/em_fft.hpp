@@ -358,6 +358,25 @@ struct FftLine {
                                              const float2* __restrict__ tw) {
     stages<1, RS...>(v, lds, tau, tws, tw);
   }
+
+  // The same transform split after its first stage, so the threads that hold
+  // a line may change between the two halves: first() runs stage 1 (no
+  // twiddles) and scatters its outputs to LDS; rest() reads them in the
+  // strided distribution of the (possibly different) owning threads and runs
+  // the remaining stages.  The caller synchronises in between.
+  template <int R1, int... RS>
+  __device__ __forceinline__ static void first(float2 (&v)[E], float2* lds, int tau) {
+    butterflies<R1, 0>(v);
+    store_stage<R1, 1>(v, lds, tau);
+  }
+  template <int R1, int... RS>
+  __device__ __forceinline__ static void rest(float2 (&v)[E], float2* lds, int tau,
+                                              const TwChain<N, T, 1, R1, RS...>& tws,
+                                              const float2* __restrict__ tw) {
+    load_strided(v, lds, tau);
+    lds_sync<SYNC>();
+    stages<R1, RS...>(v, lds, tau, tws.next, tw);
+  }
 };
 
 }  // namespace em

@@ -29,6 +29,17 @@ struct FftCfg;
                                                const float2* tw) {                  \
       FftLine<N_, T_, DIR, SYNC>::template run<__VA_ARGS__>(v, lds, tau, tws, tw);  \
     }                                                                               \
+    template <int DIR>                                                              \
+    __device__ __forceinline__ static void first(float2 (&v)[N_ / T_], float2* lds, \
+                                                 int tau) {                         \
+      FftLine<N_, T_, DIR, SYNC_WAVE>::template first<__VA_ARGS__>(v, lds, tau);    \
+    }                                                                               \
+    template <int DIR, int SYNC>                                                    \
+    __device__ __forceinline__ static void rest(float2 (&v)[N_ / T_], float2* lds,  \
+                                                int tau, const Tw& tws,             \
+                                                const float2* tw) {                 \
+      FftLine<N_, T_, DIR, SYNC>::template rest<__VA_ARGS__>(v, lds, tau, tws, tw); \
+    }                                                                               \
   };
 EM_FFT_CFG(64, 8, 8, 8)
 EM_FFT_CFG(128, 16, 8, 8, 2)
@@ -279,6 +290,72 @@ __global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) void col_pass_kernel(
 }
 
 // --------------------------------------------------------------------------
+// Column pass, transposed-exchange form of the per-slice modes (FFT_y, P or
+// conj P, IFFT_y), for column lengths held by one 64-lane wavefront.
+// 16 columns per workgroup, one wavefront per column after the first FFT stage:
+//   load (coalesced: thread = column c + 16 * row phase tau) -> stage 1 in
+//   registers -> scatter to LDS -> barrier -> wavefront w owns column w for
+//   the remaining stages, the multiply and the whole inverse transform
+//   (wave-local exchanges) -> transpose back through LDS -> barrier ->
+//   coalesced store.
+// Three workgroup barriers per probe instead of one per exchange.
+// --------------------------------------------------------------------------
+template <int NY, int COLS, unsigned MODE>
+__global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
+  static_assert(FftCfg<NY>::T == 64, "one wavefront per column");
+  constexpr int T = 64;
+  constexpr int E = NY / T;
+  constexpr int STRIDE = LineLds<NY>::size + 1;
+  extern __shared__ float2 smem[];
+  const int c = threadIdx.x % COLS, tau = threadIdx.x / COLS;  // coalesced mapping
+  const int w = threadIdx.x >> 6, lam = threadIdx.x & 63;      // column-per-wavefront mapping
+  const int x0 = blockIdx.x * COLS;
+  const int nx = a.nx;
+  const int b0 = blockIdx.y * a.pb;
+  const int b1 = min(a.nprobe, b0 + a.pb);
+  float2* ldsA = smem + c * STRIDE;
+  float2* ldsB = smem + w * STRIDE;
+  typename FftCfg<NY>::Tw tws;
+  tws.init(lam, a.tw);
+
+  // propagator of column w in the column-per-wavefront mapping: read
+  // coalesced, transposed once through LDS
+  float2 p[E];
+  {
+#pragma unroll
+    for (int e = 0; e < E; ++e) ldsA[LineLds<NY>::idx(tau + T * e)] = a.P[(long long)(tau + T * e) * nx + x0 + c];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      p[e] = ldsB[LineLds<NY>::idx(lam + T * e)];
+      if constexpr ((MODE & COL_CONJ_P) != 0) p[e].y = -p[e].y;
+    }
+    __syncthreads();
+  }
+
+  for (int b = b0; b < b1; ++b) {
+    const long long base = (long long)b * a.pstride + x0 + c;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = a.in[base + (long long)(tau + T * e) * nx];
+    FftCfg<NY>::template first<-1>(v, ldsA, tau);
+    __syncthreads();
+    FftCfg<NY>::template rest<-1, SYNC_WAVE>(v, ldsB, lam, tws, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
+    FftCfg<NY>::template run<+1, SYNC_WAVE>(v, ldsB, lam, tws, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) ldsB[LineLds<NY>::idx(lam + T * e)] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = ldsA[LineLds<NY>::idx(tau + T * e)];
+#pragma unroll
+    for (int e = 0; e < E; ++e) a.out[base + (long long)(tau + T * e) * nx] = v[e];
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
 // Mode tables: every (size, mode) the host layer launches is instantiated here.
 // --------------------------------------------------------------------------
 #define EM_ROW_MODES(X)                                                         \
@@ -379,7 +456,32 @@ static hipError_t col_attrs(void) {
                             (int)ColCfg<NY, AltCols<NY>::value>::LDS);
   EM_COL_HOT_MODES(EM_COL_ATTR_ALT)
 #undef EM_COL_ATTR_ALT
+  if constexpr (FftCfg<NY>::T == 64) {
+#define EM_COL_ATTR_T(M)                                                                 \
+    if (e == hipSuccess)                                                                 \
+      e = hipFuncSetAttribute((const void*)col_pass_t_kernel<NY, 16, (M)>,                \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                \
+                              (int)(16 * (LineLds<NY>::size + 1) * sizeof(float2)));     \
+    if (e == hipSuccess)                                                                 \
+      e = hipFuncSetAttribute((const void*)col_pass_t_kernel<NY, 8, (M)>,                 \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                \
+                              (int)(8 * (LineLds<NY>::size + 1) * sizeof(float2)));
+    EM_COL_HOT_MODES(EM_COL_ATTR_T)
+#undef EM_COL_ATTR_T
+  }
   return e;
+}
+
+bool col_transposed_available(int ny) {
+  switch (ny) {
+#define EM_CASE(N) \
+  case N:          \
+    return FftCfg<N>::T == 64;
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return false;
+  }
 }
 
 hipError_t prepare_col_kernels(int ny) {
@@ -394,8 +496,31 @@ hipError_t prepare_col_kernels(int ny) {
   }
 }
 
+template <int NY, int COLS, unsigned M>
+static hipError_t col_launch_t(const ColArgs& a, hipStream_t s) {
+  if constexpr (FftCfg<NY>::T == 64) {
+    constexpr size_t lds = (size_t)COLS * (LineLds<NY>::size + 1) * sizeof(float2);
+    const dim3 grid(a.nx / COLS, (a.nprobe + a.pb - 1) / a.pb);
+    hipLaunchKernelGGL((col_pass_t_kernel<NY, COLS, M>), grid, dim3(COLS * 64), lds, s, a);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
 template <int NY>
 static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
+  if (a.variant == COL_VARIANT_TRANSPOSED && (a.cols == 0 || a.cols == 16 || a.cols == 8)) {
+    switch (mode) {
+#define EM_COL_T_CASE(M)                                                                  \
+  case (M):                                                                               \
+    return a.cols == 8 ? col_launch_t<NY, 8, (M)>(a, s) : col_launch_t<NY, 16, (M)>(a, s);
+      EM_COL_HOT_MODES(EM_COL_T_CASE)
+#undef EM_COL_T_CASE
+      default:
+        break;
+    }
+  }
   if (a.cols == AltCols<NY>::value) {
     switch (mode) {
 #define EM_COL_ALT_CASE(M) \

@@ -87,7 +87,10 @@ struct ColArgs {
   double dg2min[kMaxDet], dg2max[kMaxDet];
   float* dpart;             // [nprobe][ndet][ntiles]
   int cols;                 // column tile of the per-slice modes: 0 = default, or col_alt_tile
+  int variant;              // per-slice modes: COL_VARIANT_BLOCK or COL_VARIANT_TRANSPOSED
 };
+
+enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };
 
 hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
 hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
@@ -98,6 +101,8 @@ bool fft_size_supported(int n);
 int row_lines_per_block(int nx);
 int col_tile(int ny);      // default columns per column-pass workgroup
 int col_alt_tile(int ny);  // the alternative tile of the per-slice column modes
+// whether the transposed-exchange column pass exists for this column length
+bool col_transposed_available(int ny);
 int col_ntiles(int nx, int ny);
 
 // --- other kernels ------------------------------------------------------------

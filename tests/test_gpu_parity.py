@@ -347,15 +347,16 @@ def test_tuning_variants_agree(em):
     leaves a valid tuning behind."""
     import torch
     rng = np.random.default_rng(13)
-    n, S, B = 256, 5, 6
+    n, S, B = 512, 5, 6
     V = torch.from_numpy(rng.uniform(0, 1, (S, n, n)).astype(np.float32)).cuda()
     with em.Plan(n, n, 20.0, 20.0, 2.0, 300.0, max_batch=B) as p:
         psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, 20, B), rng.uniform(0, 20, B))
         outs = []
         alt = 8
-        for tun in (dict(row_blocks=2048, col_blocks=512, col_tile=0, probe_group=0),
-                    dict(row_blocks=64, col_blocks=16, col_tile=alt, probe_group=2),
-                    dict(row_blocks=8192, col_blocks=4096, col_tile=alt, probe_group=5)):
+        for tun in (dict(row_blocks=2048, col_blocks=512, col_tile=0, probe_group=0, col_variant=1),
+                    dict(row_blocks=64, col_blocks=16, col_tile=alt, probe_group=2, col_variant=0),
+                    dict(row_blocks=8192, col_blocks=4096, col_tile=alt, probe_group=5, col_variant=2),
+                    dict(row_blocks=512, col_blocks=128, col_tile=0, probe_group=3, col_variant=2)):
             p.set_tuning(**tun)
             psi = psi0.clone()
             p.multislice_forward(psi, V=V)
