@@ -35,6 +35,7 @@ EXPORTS = [
     "em_detector_sums", "em_ctf_image", "em_diffraction", "em_stem", "em_thickness",
     "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
+    "em_plan_set_band_skip",
 ]
 
 
@@ -104,6 +105,7 @@ _SIGS = {
     "em_plan_get_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),
     "em_plan_set_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),
     "em_plan_autotune": ([_P, _I, _I, _PD], _I),
+    "em_plan_set_band_skip": ([_P, _D, _PI, _PI], _I),
 }
 
 _lib = None

@@ -71,6 +71,10 @@ struct em_plan_s {
   double vr_factor = 0, gx0 = 0, gy0 = 0;
   GridDev gdev{};
   em_tuning tune{2048, 512, 0, 0, 0};
+  // band-limit skipping: columns kx in [skip_lo, skip_hi] of the propagator
+  // are below band_skip (relative) for every ky; skip_lo > skip_hi: none
+  double band_skip = 1e-9;
+  int skip_lo = 1, skip_hi = 0;
   std::string err;
   // spline settings and element overrides
   int nR = 128;
@@ -144,6 +148,28 @@ int col_pb(const em_plan p, int B) {
   return (B + groups - 1) / groups;
 }
 
+// Columns whose band-width-limit factor 1/(1+exp(alpha (g^2 - gl^2))) is at
+// most band_skip for every ky (the factor is largest at ky = 0): the
+// propagator there is below band_skip / N, and a wave component passing it
+// changes the result by less than band_skip relative -- under FP32 resolution
+// for band_skip <= 1e-9.  They form the range |kx_s| >= K of the unshifted
+// index, i.e. kx in [K, nx - K].
+void compute_band_skip(em_plan p) {
+  p->skip_lo = 1;
+  p->skip_hi = 0;
+  if (!p->grid.bwl || !(p->band_skip > 0) || std::fabs(p->grid.dz) <= 1e-5) return;
+  const int nx = p->grid.nx;
+  for (int k = 1; k <= nx / 2; ++k) {
+    const double gx = k * p->gdev.dgx;
+    const double f = 1.0 / (1.0 + std::exp(p->gdev.alpha * (gx * gx - p->gdev.gl2_max)));
+    if (f <= p->band_skip) {
+      p->skip_lo = k;
+      p->skip_hi = nx - k;
+      return;
+    }
+  }
+}
+
 RowArgs row_args(em_plan p, const float2* in, float2* out, int B) {
   RowArgs a{};
   a.in = in;
@@ -155,6 +181,19 @@ RowArgs row_args(em_plan p, const float2* in, float2* out, int B) {
   a.pb = row_pb(p, B);
   a.tw = p->tw_x.as<float2>();
   a.wgt = 1.f;
+  a.skip_lo = p->skip_lo;
+  a.skip_hi = p->skip_hi;
+  a.skip_in = 0;
+  a.skip_out = 0;
+  return a;
+}
+
+// the row pass of a slice step: input from a propagator column pass (or the
+// probe build) and / or output to one
+RowArgs row_args_skip(em_plan p, const float2* in, float2* out, int B, bool skip_in, bool skip_out) {
+  RowArgs a = row_args(p, in, out, B);
+  a.skip_in = skip_in ? 1 : 0;
+  a.skip_out = skip_out ? 1 : 0;
   return a;
 }
 
@@ -172,7 +211,17 @@ ColArgs col_args(em_plan p, const float2* in, float2* out, int B) {
   a.grid = p->gdev;
   a.cols = p->tune.col_tile;
   const bool tr = col_transposed_available(p->grid.ny) && (a.cols == 0 || a.cols == 16 || a.cols == 8);
-  a.variant = (p->tune.col_variant == 1 || !tr) ? COL_VARIANT_BLOCK : COL_VARIANT_TRANSPOSED;
+  a.variant = (p->tune.col_variant == 2 && tr) ? COL_VARIANT_TRANSPOSED : COL_VARIANT_BLOCK;
+  a.skip_lo = 1;
+  a.skip_hi = 0;
+  return a;
+}
+
+// a propagator column pass whose band-limited columns are skipped
+ColArgs col_args_skip(em_plan p, const float2* in, float2* out, int B) {
+  ColArgs a = col_args(p, in, out, B);
+  a.skip_lo = p->skip_lo;
+  a.skip_hi = p->skip_hi;
   return a;
 }
 
@@ -189,7 +238,7 @@ em_status ensure_specimen_potential(em_plan p);
 em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
                         float2* ckpt, long long ckpt_slice_stride, int every, bool in_mixed_layout) {
   for (int s = 0; s < S; ++s) {
-    RowArgs r = row_args(p, wave, wave, nb);
+    RowArgs r = row_args_skip(p, wave, wave, nb, s > 0 || in_mixed_layout, s < S - 1);
     unsigned m = ROW_POST_FFT;
     if (s > 0 || in_mixed_layout) m |= ROW_PRE_IFFT;
     if (V) {
@@ -204,7 +253,7 @@ em_status forward_group(em_plan p, const float* V, const float2* t, int S, float
       r.ckpt = ckpt + (long long)(s / every) * ckpt_slice_stride;
     }
     EM_TRY(row(p, m, r));
-    ColArgs c = col_args(p, wave, wave, nb);
+    ColArgs c = col_args_skip(p, wave, wave, nb);
     EM_TRY(col(p, s < S - 1 ? (COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE) : (COL_PRE_FFT | COL_STORE), c));
   }
   return EM_OK;
@@ -253,7 +302,18 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
     EM_HIP(raw, launch_propagator(p->gdev, w, p->gx0, p->gy0, grid->bwl, p->P.as<float2>(), nullptr));
   }
   EM_HIP(raw, hipDeviceSynchronize());
+  compute_band_skip(raw);
   *out = p.release();
+  return EM_OK;
+}
+
+em_status em_plan_set_band_skip(em_plan plan, double threshold, int* skip_lo, int* skip_hi) {
+  if (!plan || threshold < 0 || threshold > 1e-6) return EM_ERR_INVALID;
+  plan->band_skip = threshold;
+  compute_band_skip(plan);
+  plan->graphs.clear();
+  if (skip_lo) *skip_lo = plan->skip_lo;
+  if (skip_hi) *skip_hi = plan->skip_hi;
   return EM_OK;
 }
 
@@ -589,7 +649,7 @@ em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S,
         float2* sg = p->seg.as<float2>();
         EM_HIP(p, hipMemcpyAsync(sw, ck + (long long)j * BN, sizeof(float2) * BN, hipMemcpyDeviceToDevice, p->stream));
         for (int s = s0; s <= k; ++s) {
-          RowArgs r = row_args(p, sw, sw, nprobe);
+          RowArgs r = row_args_skip(p, sw, sw, nprobe, s > s0, s < k);
           r.ckpt = sg + (long long)(s - s0) * BN;
           unsigned m = ROW_CKPT_STORE | (s > s0 ? ROW_PRE_IFFT : 0u);
           if (s < k) {
@@ -597,12 +657,12 @@ em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S,
             if (V) r.V = V + (long long)s * N; else r.t = tc + (long long)s * N;
           }
           EM_TRY(row(p, m, r));
-          if (s < k) EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, sw, sw, nprobe)));
+          if (s < k) EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, sw, sw, nprobe)));
         }
       }
       ckk = p->seg.as<float2>() + (long long)(k - s0) * BN;
     }
-    RowArgs r = row_args(p, lam, lam, nprobe);
+    RowArgs r = row_args_skip(p, lam, lam, nprobe, k < S - 1, k > 0);
     r.ckpt = const_cast<float2*>(ckk);
     r.gpart = nparts == 1 ? gtc + (long long)k * N : p->gpart.as<float2>();
     unsigned m = ROW_PRE_IFFT | ROW_GRAD;
@@ -615,7 +675,7 @@ em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S,
     EM_TRY(row(p, m, r));
     if (nparts > 1)
       EM_HIP(p, launch_grad_reduce(N, nparts, p->gpart.as<float2>(), gtc + (long long)k * N, p->stream));
-    if (k > 0) EM_TRY(col(p, COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE, col_args(p, lam, lam, nprobe)));
+    if (k > 0) EM_TRY(col(p, COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, lam, lam, nprobe)));
   }
   if (V && gV) EM_HIP(p, launch_grad_v(gtc, V, 1.f, N * S, gV, p->stream));
   return EM_OK;
@@ -713,10 +773,10 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     c.ey = p->ey.as<float2>() + (long long)g0 * p->grid.ny;
     EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
     for (int s = 0; s < S; ++s) {
-      RowArgs r = row_args(p, wave, wave, ng);
+      RowArgs r = row_args_skip(p, wave, wave, ng, true, s < S - 1);
       r.t = p->Tstore.as<float2>() + (long long)s * N;
       EM_TRY(row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r));
-      ColArgs cc = col_args(p, wave, wave, ng);
+      ColArgs cc = col_args_skip(p, wave, wave, ng);
       if (s < S - 1) {
         EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc));
       } else {
@@ -901,11 +961,11 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
     const int B = nprobe[r];
     int io = 0;
     for (int s = 0; s < S[r] && st == EM_OK; ++s) {
-      RowArgs ra = row_args(p, wv, wv, B);
+      const bool rec = ((s + 1) % stride == 0) || s == S[r] - 1;
+      RowArgs ra = row_args_skip(p, wv, wv, B, s > 0, !rec && s < S[r] - 1);
       ra.V = V[r] + (long long)s * N;
       st = row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, ra);
       if (st != EM_OK) break;
-      const bool rec = ((s + 1) % stride == 0) || s == S[r] - 1;
       if (rec) {
         ColArgs cd = col_args(p, wv, nullptr, B);
         cd.ndet = ndet;
@@ -922,7 +982,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
         ++io;
       }
       if (st == EM_OK && s < S[r] - 1)
-        st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, wv, wv, B));
+        st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, wv, wv, B));
     }
     if (st == EM_OK) st = p->hip(hipEventRecord(done[r], br), "record");
     if (st == EM_OK && r > 0) st = p->hip(hipStreamWaitEvent(p0->gstream, done[r], 0), "join");
@@ -952,9 +1012,9 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
   // the row pass as em_stem runs it: transmission precomputed per slice
   EM_HIP(p, p->tmp.ensure(sizeof(float2) * p->N()));
   EM_HIP(p, launch_transmission(V, p->tmp.as<float2>(), p->N(), p->stream));
-  RowArgs r = row_args(p, w, w, nprobe);
+  RowArgs r = row_args_skip(p, w, w, nprobe, true, true);
   r.t = p->tmp.as<float2>();
-  const ColArgs c = col_args(p, w, w, nprobe);
+  const ColArgs c = col_args_skip(p, w, w, nprobe);
   em_status st = EM_OK;
   // warm up both kernels once
   st = row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r);

@@ -121,7 +121,11 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
       for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
     } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = a.in[base + tau + T * e];
+      for (int e = 0; e < E; ++e) {
+        const int kx = tau + T * e;
+        const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
+        v[e] = skip ? make_float2(0.f, 0.f) : a.in[base + kx];
+      }
     }
     if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
@@ -163,7 +167,10 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.out[base + tau + T * e] = v[e];
+      for (int e = 0; e < E; ++e) {
+        const int kx = tau + T * e;
+        if (!(a.skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) a.out[base + kx] = v[e];
+      }
     }
   }
   if constexpr ((MODE & ROW_GRAD) != 0) {
@@ -193,7 +200,10 @@ __global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) void col_pass_kernel(
   extern __shared__ float2 smem[];
   const int c = threadIdx.x % COLS;
   const int tau = threadIdx.x / COLS;
-  const int x = blockIdx.x * COLS + c;
+  int tile = blockIdx.x;
+  if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0)
+    if (a.tile_lo <= a.tile_hi && tile >= a.tile_lo) tile += a.tile_hi - a.tile_lo + 1;
+  const int x = tile * COLS + c;
   const int nx = a.nx;
   const int b0 = blockIdx.y * a.pb;
   const int b1 = min(a.nprobe, b0 + a.pb);
@@ -309,7 +319,9 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   extern __shared__ float2 smem[];
   const int c = threadIdx.x % COLS, tau = threadIdx.x / COLS;  // coalesced mapping
   const int w = threadIdx.x >> 6, lam = threadIdx.x & 63;      // column-per-wavefront mapping
-  const int x0 = blockIdx.x * COLS;
+  int tile = blockIdx.x;
+  if (a.tile_lo <= a.tile_hi && tile >= a.tile_lo) tile += a.tile_hi - a.tile_lo + 1;
+  const int x0 = tile * COLS;
   const int nx = a.nx;
   const int b0 = blockIdx.y * a.pb;
   const int b1 = min(a.nprobe, b0 + a.pb);
@@ -431,10 +443,25 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   }
 }
 
+// Tiles of COLS columns wholly inside [skip_lo, skip_hi] (propagator modes only).
+template <int COLS>
+static int skipped_tiles(unsigned mode, ColArgs& a) {
+  a.tile_lo = 1;
+  a.tile_hi = 0;
+  if ((mode & (COL_MUL_P | COL_CONJ_P)) == 0 || a.skip_lo > a.skip_hi) return 0;
+  const int lo = (a.skip_lo + COLS - 1) / COLS, hi = (a.skip_hi + 1) / COLS - 1;
+  if (lo > hi) return 0;
+  a.tile_lo = lo;
+  a.tile_hi = hi;
+  return hi - lo + 1;
+}
+
 template <int NY, int COLS, unsigned M>
-static hipError_t col_launch_one(const ColArgs& a, hipStream_t s) {
+static hipError_t col_launch_one(const ColArgs& args, hipStream_t s) {
   using C = ColCfg<NY, COLS>;
-  const dim3 grid(a.nx / C::COLS, (a.nprobe + a.pb - 1) / a.pb);
+  ColArgs a = args;
+  const int skipped = skipped_tiles<COLS>(M, a);
+  const dim3 grid(a.nx / C::COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
   hipLaunchKernelGGL((col_pass_kernel<NY, COLS, M>), grid, dim3(C::THREADS), C::LDS, s, a);
   return hipGetLastError();
 }
@@ -497,10 +524,12 @@ hipError_t prepare_col_kernels(int ny) {
 }
 
 template <int NY, int COLS, unsigned M>
-static hipError_t col_launch_t(const ColArgs& a, hipStream_t s) {
+static hipError_t col_launch_t(const ColArgs& args, hipStream_t s) {
   if constexpr (FftCfg<NY>::T == 64) {
     constexpr size_t lds = (size_t)COLS * (LineLds<NY>::size + 1) * sizeof(float2);
-    const dim3 grid(a.nx / COLS, (a.nprobe + a.pb - 1) / a.pb);
+    ColArgs a = args;
+    const int skipped = skipped_tiles<COLS>(M, a);
+    const dim3 grid(a.nx / COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
     hipLaunchKernelGGL((col_pass_t_kernel<NY, COLS, M>), grid, dim3(COLS * 64), lds, s, a);
     return hipGetLastError();
   } else {

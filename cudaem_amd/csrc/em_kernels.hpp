@@ -51,6 +51,11 @@ struct RowArgs {
   float* o1;                // [nprobe][ny][nx] intensity or real part
   float* o2;                // [nprobe][ny][nx] imaginary part
   float wgt;
+  // Band-limit skipping: columns kx in [skip_lo, skip_hi] carry nothing the
+  // next column pass keeps (the propagator is below the skip threshold there).
+  // skip_in: those input columns are taken as zero and not read;
+  // skip_out: those output columns are not written.
+  int skip_lo, skip_hi, skip_in, skip_out;
 };
 
 // --- column pass ------------------------------------------------------------
@@ -88,6 +93,10 @@ struct ColArgs {
   float* dpart;             // [nprobe][ndet][ntiles]
   int cols;                 // column tile of the per-slice modes: 0 = default, or col_alt_tile
   int variant;              // per-slice modes: COL_VARIANT_BLOCK or COL_VARIANT_TRANSPOSED
+  // per-slice modes only: column tiles wholly inside [skip_lo, skip_hi] are
+  // neither read nor written (set from skip_lo/hi by the launcher)
+  int skip_lo, skip_hi;
+  int tile_lo, tile_hi;
 };
 
 enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };

@@ -115,6 +115,14 @@ class Plan:
         t = C.em_tuning(**cur)
         C.check(C.lib().em_plan_set_tuning(self._h, ctypes.byref(t)), self._h, "set_tuning")
 
+    def set_band_skip(self, threshold=1e-9):
+        """Skip propagator columns below threshold (relative); 0 disables.
+        Returns the skipped column range (lo, hi), lo > hi when none."""
+        lo, hi = ctypes.c_int(), ctypes.c_int()
+        C.check(C.lib().em_plan_set_band_skip(self._h, float(threshold), ctypes.byref(lo), ctypes.byref(hi)),
+                self._h, "set_band_skip")
+        return lo.value, hi.value
+
     def autotune(self, nprobe, nslices=4):
         """Times candidate launch shapes on this grid and keeps the fastest;
         returns (tuning, ms per probe-slice)."""

@@ -105,6 +105,14 @@ em_status em_plan_set_tuning(em_plan plan, const em_tuning* tuning);
  * HIP events and keeps the fastest; best_ms_per_probe_slice receives its
  * time.  Allocates and frees its own scratch. */
 em_status em_plan_autotune(em_plan plan, int nprobe, int nslices, double* best_ms_per_probe_slice);
+/* Band-limit skipping (no reference counterpart).  With the band-width limit
+ * on, the propagator of every column kx with bwl factor <= threshold at ky = 0
+ * is below threshold / N in every row, so the column passes of the slice loop
+ * skip those columns and the row passes neither read nor write them (except
+ * where the full exit wave is formed).  The skipped contributions are below
+ * threshold relative; default 1e-9 (under FP32 resolution); 0 disables;
+ * at most 1e-6.  skip_lo / skip_hi receive the skipped column range. */
+em_status em_plan_set_band_skip(em_plan plan, double threshold, int* skip_lo, int* skip_hi);
 /* Last error message of this plan (static storage, valid until the next call). */
 const char* em_plan_last_error(em_plan plan);
 int em_fft_size_supported(int n);
