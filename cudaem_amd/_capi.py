@@ -54,7 +54,7 @@ class em_lens(ctypes.Structure):
 
 class em_tuning(ctypes.Structure):
     _fields_ = [("row_blocks", ctypes.c_int), ("col_blocks", ctypes.c_int), ("col_tile", ctypes.c_int),
-                ("probe_group", ctypes.c_int), ("col_variant", ctypes.c_int)]
+                ("probe_group", ctypes.c_int), ("col_variant", ctypes.c_int), ("row_variant", ctypes.c_int)]
 
 
 class em_atoms(ctypes.Structure):

@@ -70,7 +70,7 @@ struct em_plan_s {
   Constants k{};
   double vr_factor = 0, gx0 = 0, gy0 = 0;
   GridDev gdev{};
-  em_tuning tune{2048, 512, 0, 0, 0};
+  em_tuning tune{2048, 1024, 0, 0, 0, 0};
   // band-limit skipping: columns kx in [skip_lo, skip_hi] of the propagator
   // are below band_skip (relative) for every ky; skip_lo > skip_hi: none
   double band_skip = 1e-9;
@@ -234,6 +234,12 @@ em_status col(em_plan p, unsigned mode, const ColArgs& a) {
 
 em_status ensure_specimen_potential(em_plan p);
 
+// the interior slice step on precomputed transmissions may read t per probe
+// instead of holding it in registers (tuning row_variant 1)
+unsigned row_t_mode(em_plan p, unsigned m) {
+  return (m == (ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT) && p->tune.row_variant == 1) ? (m | ROW_T_STREAM) : m;
+}
+
 // Fused forward multislice over nb probes at wave (real space in, Fourier out).
 em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
                         float2* ckpt, long long ckpt_slice_stride, int every, bool in_mixed_layout) {
@@ -252,7 +258,7 @@ em_status forward_group(em_plan p, const float* V, const float2* t, int S, float
       m |= ROW_CKPT_STORE;
       r.ckpt = ckpt + (long long)(s / every) * ckpt_slice_stride;
     }
-    EM_TRY(row(p, m, r));
+    EM_TRY(row(p, row_t_mode(p, m), r));
     ColArgs c = col_args_skip(p, wave, wave, nb);
     EM_TRY(col(p, s < S - 1 ? (COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE) : (COL_PRE_FFT | COL_STORE), c));
   }
@@ -354,7 +360,7 @@ em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
 
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
   if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0 || t->col_variant < 0 ||
-      t->col_variant > 2)
+      t->col_variant > 2 || t->row_variant < 0 || t->row_variant > 1)
     return EM_ERR_INVALID;
   if (t->col_tile != 0 && t->col_tile != col_tile(plan->grid.ny) &&
       (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0) &&
@@ -775,7 +781,7 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     for (int s = 0; s < S; ++s) {
       RowArgs r = row_args_skip(p, wave, wave, ng, true, s < S - 1);
       r.t = p->Tstore.as<float2>() + (long long)s * N;
-      EM_TRY(row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r));
+      EM_TRY(row(p, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), r));
       ColArgs cc = col_args_skip(p, wave, wave, ng);
       if (s < S - 1) {
         EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc));
@@ -1017,10 +1023,11 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
   const ColArgs c = col_args_skip(p, w, w, nprobe);
   em_status st = EM_OK;
   // warm up both kernels once
-  st = row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r);
+  const unsigned rm = row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT);
+  st = row(p, rm, r);
   if (st == EM_OK) st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
   if (st == EM_OK) st = p->hip(hipEventRecord(e[0], p->stream), "event");
-  for (int i = 0; i < iters && st == EM_OK; ++i) st = row(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT, r);
+  for (int i = 0; i < iters && st == EM_OK; ++i) st = row(p, rm, r);
   if (st == EM_OK) st = p->hip(hipEventRecord(e[1], p->stream), "event");
   for (int i = 0; i < iters && st == EM_OK; ++i)
     st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
@@ -1082,22 +1089,23 @@ extern "C" em_status em_host_constants(double e0_kv, double* lambda, double* gam
 extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double* best) {
   if (!p || nprobe <= 0 || nslices <= 0) return EM_ERR_INVALID;
   const long long N = p->N();
-  float* V = nullptr;
+  // timed on precomputed transmissions, the form the STEM slice loop runs
+  float2* T = nullptr;
   float2* w = nullptr;
-  EM_HIP(p, hipMalloc(&V, sizeof(float) * N * nslices));
+  EM_HIP(p, hipMalloc(&T, sizeof(float2) * N * nslices));
   if (hipMalloc(&w, sizeof(float2) * N * nprobe) != hipSuccess) {
-    (void)hipFree(V);
+    (void)hipFree(T);
     return p->fail(EM_ERR_ALLOC, "autotune scratch");
   }
   hipEvent_t e0, e1;
-  em_status st = p->hip(hipMemsetAsync(V, 0, sizeof(float) * N * nslices, p->stream), "memset");
+  em_status st = p->hip(hipMemsetAsync(T, 0, sizeof(float2) * N * nslices, p->stream), "memset");
   if (st == EM_OK) st = p->hip(hipMemsetAsync(w, 0, sizeof(float2) * N * nprobe, p->stream), "memset");
   if (st == EM_OK) st = p->hip(hipEventCreate(&e0), "event");
   if (st == EM_OK) st = p->hip(hipEventCreate(&e1), "event");
   auto measure = [&](double* ms) -> em_status {
-    EM_TRY(em_multislice_forward(p, V, nullptr, nslices, w, nprobe, nullptr, 1));  // warm
+    EM_TRY(em_multislice_forward(p, nullptr, T, nslices, w, nprobe, nullptr, 1));  // warm
     EM_HIP(p, hipEventRecord(e0, p->stream));
-    for (int i = 0; i < 2; ++i) EM_TRY(em_multislice_forward(p, V, nullptr, nslices, w, nprobe, nullptr, 1));
+    for (int i = 0; i < 2; ++i) EM_TRY(em_multislice_forward(p, nullptr, T, nslices, w, nprobe, nullptr, 1));
     EM_HIP(p, hipEventRecord(e1, p->stream));
     EM_HIP(p, hipEventSynchronize(e1));
     float t = 0;
@@ -1113,10 +1121,16 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   const int colb[] = {256, 512, 1024, 2048};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
+  const int row_variants[] = {0, 1};
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
-    for (int par = 0; par < 5 && st == EM_OK; ++par) {
-      const int* cand = par == 0 ? groups : par == 1 ? rowb : par == 2 ? colb : par == 3 ? tiles : variants;
-      const int nc = par == 0 ? 5 : (par == 3 || par == 4) ? 2 : 4;
+    for (int par = 0; par < 6 && st == EM_OK; ++par) {
+      const int* cand = par == 0   ? groups
+                        : par == 1 ? rowb
+                        : par == 2 ? colb
+                        : par == 3 ? tiles
+                        : par == 4 ? variants
+                                   : row_variants;
+      const int nc = par == 0 ? 5 : par >= 3 ? 2 : 4;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;
@@ -1128,6 +1142,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
           t.col_tile = cand[i];
         }
         if (par == 4) t.col_variant = cand[i];
+        if (par == 5) t.row_variant = cand[i];
         if (par == 0 && cand[i] > nprobe) continue;
         const em_tuning keep = p->tune;
         p->tune = t;
@@ -1146,7 +1161,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(p->stream);
-  (void)hipFree(V);
+  (void)hipFree(T);
   (void)hipFree(w);
   if (best) *best = cur;
   return st;

@@ -74,8 +74,16 @@ struct ColCfg {
 // --------------------------------------------------------------------------
 // Row pass
 // --------------------------------------------------------------------------
+// Register budget: the streamed-transmission form is held to 128 VGPRs
+// (4 waves per SIMD); the others may use more and run 3 waves per SIMD.
+template <unsigned MODE>
+struct RowOcc {
+  static constexpr int waves = (MODE & ROW_T_STREAM) ? 4 : 1;
+};
+
 template <int NX, unsigned MODE>
-__global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(RowOcc<MODE>::waves)))
+void row_pass_kernel(RowArgs a) {
   constexpr int T = FftCfg<NX>::T;
   constexpr int E = NX / T;
   constexpr int LINES = kRowThreads / T;
@@ -92,8 +100,8 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
   tws.init(tau, a.tw);
 
   // transmission values of this thread's elements, reused for every probe
-  float2 t[E];
-  if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0) {
+  float2 t[(MODE & ROW_T_STREAM) ? 1 : E];
+  if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0 && (MODE & ROW_T_STREAM) == 0) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const long long i = row + tau + T * e;
@@ -141,7 +149,10 @@ __global__ __launch_bounds__(kRowThreads) void row_pass_kernel(RowArgs a) {
         g[e].y += p.x * v[e].y - p.y * v[e].x;
       }
     }
-    if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0) {
+    if constexpr ((MODE & ROW_T_STREAM) != 0) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = cmul(v[e], a.t[row + tau + T * e]);
+    } else if constexpr ((MODE & (ROW_MUL_V | ROW_MUL_T)) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = cmul(v[e], t[e]);
     }
@@ -191,8 +202,11 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
   return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
 }
 
+// At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
+// 8-column tile then share a CU, one loading while the other computes.
 template <int NY, int COLS_, unsigned MODE>
-__global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) void col_pass_kernel(ColArgs a) {
+__global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+void col_pass_kernel(ColArgs a) {
   using C = ColCfg<NY, COLS_>;
   constexpr int T = C::T;
   constexpr int E = NY / T;
@@ -378,6 +392,7 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   X(ROW_IN_PLANE | ROW_MUL_V | ROW_POST_FFT)                                    \
   X(ROW_MUL_T | ROW_POST_FFT)                                                   \
   X(ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT)                                    \
+  X(ROW_PRE_IFFT | ROW_MUL_T | ROW_T_STREAM | ROW_POST_FFT)                     \
   X(ROW_PRE_IFFT | ROW_CKPT_STORE | ROW_MUL_T | ROW_POST_FFT)                   \
   X(ROW_CKPT_STORE | ROW_MUL_T | ROW_POST_FFT)                                  \
   X(ROW_PRE_IFFT | ROW_GRAD | ROW_MUL_T | ROW_CONJ | ROW_POST_FFT)              \

@@ -36,6 +36,8 @@ enum RowMode : unsigned {
   ROW_OUT_ABS2 = 1u << 8,    // store |.|^2 (FP32) instead of the complex wave
   ROW_OUT_SPLIT = 1u << 9,   // store re / im planes
   ROW_OUT_ACCUM = 1u << 10,  // accumulate weight * output into the FP32 planes
+  ROW_T_STREAM = 1u << 11,   // read t for every probe instead of holding it in
+                             // registers (frees 2E VGPRs for a fourth wave per SIMD)
 };
 
 struct RowArgs {

@@ -94,9 +94,11 @@ em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
  * through all slices (0 = as many as fit 128 MiB, the working set the
  * last-level cache can hold); col_variant: exchange scheme of the per-slice
  * column pass (0 = automatic, 1 = workgroup barrier per exchange,
- * 2 = one wavefront per column after the first stage). */
+ * 2 = one wavefront per column after the first stage); row_variant: row
+ * pass of the STEM slice loop (0 = transmission held in registers, 1 = read
+ * per probe, four waves per SIMD). */
 typedef struct em_tuning {
-  int row_blocks, col_blocks, col_tile, probe_group, col_variant;
+  int row_blocks, col_blocks, col_tile, probe_group, col_variant, row_variant;
 } em_tuning;
 em_status em_plan_get_tuning(em_plan plan, em_tuning* tuning);
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* tuning);

@@ -366,8 +366,17 @@ def test_tuning_variants_agree(em):
         psi = psi0.clone()
         p.multislice_forward(psi, V=V)
         outs.append(_np(psi))
+        # precomputed transmissions: held in registers or read per probe
+        t = torch.polar(torch.ones_like(V), V * 0.7).contiguous()
+        touts = []
+        for rv in (0, 1):
+            p.set_tuning(row_blocks=2048, col_blocks=512, col_tile=0, probe_group=0, col_variant=1, row_variant=rv)
+            psi = psi0.clone()
+            p.multislice_forward(psi, t=t)
+            touts.append(_np(psi))
     for o in outs[1:]:
         assert rel_l2(o, outs[0]) < 1e-6
+    assert rel_l2(touts[1], touts[0]) < 1e-6
 
 
 def test_band_skip_is_below_fp32_resolution(em):
