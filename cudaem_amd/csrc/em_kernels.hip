@@ -53,14 +53,17 @@ EM_FFT_CFG(2048, 128, 16, 16, 8)
 constexpr int kRowThreads = 256;
 
 // Columns per column-pass workgroup: the default for a column length N, and
-// the alternative the autotuner may pick for the per-slice modes.
+// the alternative the autotuner may pick for the per-slice modes. Eight
+// columns (a 64 KiB + pad LDS line buffer) let two workgroups share a CU, so
+// one loads while the other transforms; XCD-banded tile order keeps the
+// half-segment reads that costs from being fetched twice.
 template <int N>
 struct DefaultCols {
-  static constexpr int value = (FftCfg<N>::T >= 128) ? 8 : 16;
+  static constexpr int value = 8;
 };
 template <int N>
 struct AltCols {
-  static constexpr int value = DefaultCols<N>::value / 2;
+  static constexpr int value = (FftCfg<N>::T >= 128) ? 4 : 16;
 };
 template <int N, int COLS_ = DefaultCols<N>::value>
 struct ColCfg {
@@ -215,6 +218,17 @@ void col_pass_kernel(ColArgs a) {
   const int c = threadIdx.x % COLS;
   const int tau = threadIdx.x / COLS;
   int tile = blockIdx.x;
+  if constexpr (COLS < 16) {
+    // Workgroups reach the eight XCDs round robin by linear id. Narrow tiles
+    // share 128-byte row segments with their neighbours: give each XCD one
+    // contiguous band of gridDim.x / 8 tiles (ids r, r+8, r+16, ...), so a
+    // segment is fetched into one L2 instead of several, and rows are read
+    // in runs of up to 1 KiB per XCD.
+    if (gridDim.x % 8 == 0) {
+      const int K = gridDim.x / 8;
+      tile = (tile % 8) * K + tile / 8;
+    }
+  }
   if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0)
     if (a.tile_lo <= a.tile_hi && tile >= a.tile_lo) tile += a.tile_hi - a.tile_lo + 1;
   const int x = tile * COLS + c;
