@@ -247,7 +247,7 @@ def main():
     band = plan.set_band_skip(0.0 if args.no_band_skip else 1e-9)
     tune_ms = None
     if not args.no_autotune:
-        tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=4)
+        tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=8)
         if args.group:
             plan.set_tuning(probe_group=args.group)
     lens = {"outer_aper_mrad": ALPHA}
@@ -282,7 +282,7 @@ def main():
     # dominant kernels at the real shape: one group of probes, one slice
     N = N_GRID * N_GRID
     tun = plan.tuning()
-    G = min(args.batch, tun["probe_group"] if tun["probe_group"] > 0 else max(1, (128 << 20) // (N * 8)))
+    G = plan.group_size(min(args.batch, npos))
     V1 = plan.potential_slices(S // 2, 1)[0].contiguous()
     wave = torch.zeros((G, N_GRID, N_GRID), dtype=torch.complex64, device="cuda")
     wave[:, 0, 0] = 1.0

@@ -898,6 +898,26 @@ hipError_t launch_fill(float2* p, long long n, float2 v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Unit phasors with hashed phases: timing data with realistic bit activity
+// (all-zero buffers run at higher clocks than real waves do).
+__global__ void phase_fill_kernel(float2* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    unsigned h = (unsigned)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    float sn, cs;
+    sincosf((float)h * 1.4629180792671596e-9f, &sn, &cs);  // 2 pi / 2^32
+    p[i] = make_float2(cs, sn);
+  }
+}
+
+hipError_t launch_phase_fill(float2* p, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(phase_fill_kernel, dim3(2048), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
 // Quadrant swap of even-sized planes (fft_shift, device_kernels.cuh:342-362),
 // out of place so in and out may differ.
 __global__ void fft_shift_kernel(const float* in, float* out, int nx, int ny, long long nb, float w,

@@ -132,6 +132,7 @@ hipError_t launch_det_reduce(int nprobe, int ndet, int ntiles, const float* part
 hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g,
                               hipStream_t s);
 hipError_t launch_fill(float2* p, long long n, float2 v, hipStream_t s);
+hipError_t launch_phase_fill(float2* p, long long n, hipStream_t s);
 hipError_t launch_fft_shift(const float* in, float* out, int nx, int ny, long long nbatch, float w,
                             int accumulate, hipStream_t s);
 hipError_t launch_abs2(const float2* in, float* out, long long n, float w, int accumulate,

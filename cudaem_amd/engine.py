@@ -104,6 +104,12 @@ class Plan:
     def set_probe_group(self, g):
         C.check(C.lib().em_plan_set_probe_group(self._h, int(g)), self._h)
 
+    def group_size(self, nprobe):
+        """Probes per group the fused passes use for a batch of nprobe probes."""
+        g = ctypes.c_int(0)
+        C.check(C.lib().em_plan_group_size(self._h, int(nprobe), ctypes.byref(g)), self._h)
+        return g.value
+
     def tuning(self):
         t = C.em_tuning()
         C.check(C.lib().em_plan_get_tuning(self._h, ctypes.byref(t)), self._h)

@@ -86,6 +86,9 @@ em_status em_plan_set_stream(em_plan plan, void* hip_stream);
 em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double* gamma);
 /* Group of probes a fused pass keeps together (0 = automatic). */
 em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
+/* Probes per group the plan uses for a batch of nprobe probes: the target
+ * group size, evened out over the batch's groups (the last may be smaller). */
+em_status em_plan_group_size(em_plan plan, int nprobe, int* probes_per_group);
 /* Launch shape of the fused passes (no reference counterpart: the reference
  * relies on cuFFT's planner).  row_blocks / col_blocks: target workgroups of a
  * row / column pass (the probes each workgroup processes follow from it);
