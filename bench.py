@@ -33,6 +33,11 @@ SCAN = 64
 U_RMS = 0.08
 SEED = 1234
 HBM_PEAK = 8.0e12
+# HBM traffic per probe of one launch of the fused passes at 1024^2, groups of
+# 40 probes, band skip 1e-9, row_blocks 4096 / col_blocks 8192: rocprofv3 --pmc
+# FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000), factors from a 1 GiB device
+# copy; profiles/batched_stem_1024_hbm_traffic.md
+PMC_BYTES_PER_PROBE = {"column pass": (280.6 + 247.7) / 40 * 2**20, "row pass": (305.2 + 241.0) / 40 * 2**20}
 
 
 def parse():
@@ -121,6 +126,23 @@ def plain_path(em, torch, at, xs, ys, n_probe):
     dt = time.perf_counter() - t0
     plan.close()
     return done / dt
+
+
+def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
+    """Achievable device-copy bandwidth (read + write bytes per second) of a
+    1 GiB float32 buffer of random values, timed with events on torch's stream."""
+    a = torch.randn(nbytes // 4, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    bw = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3)
+    del a, b
+    return bw
 
 
 def adjoint_ratio(em, torch, n=512, S=40, scan=16, reps=3):
@@ -279,18 +301,33 @@ def main():
                           "probes_per_step": npos, "tuning": plan.tuning()}), flush=True)
         return
 
-    # dominant kernels at the real shape: one group of probes, one slice
+    # Dominant kernels at the real shape: one batch of the same STEM slice
+    # loop run eagerly, every interior row / column pass launch bracketed by
+    # HIP events on the plan's stream; averages over all those launches.
     N = N_GRID * N_GRID
     tun = plan.tuning()
-    G = plan.group_size(min(args.batch, npos))
-    V1 = plan.potential_slices(S // 2, 1)[0].contiguous()
-    wave = torch.zeros((G, N_GRID, N_GRID), dtype=torch.complex64, device="cuda")
-    wave[:, 0, 0] = 1.0
-    row_ms, col_ms = plan.time_passes(V1, wave, iters=20)
-    row_bytes = 16 * N * G + 4 * N
-    col_bytes = 16 * N * G + 8 * N
+    nb = min(args.batch, npos)
+    G = plan.group_size(nb)
+    probes_per_launch = nb / -(-nb // G)
+    tmp = torch.zeros((1, nb), dtype=torch.float32, device="cuda")
+    plan.set_pass_timing(True)
+    plan.stem(lens, xs[:nb], ys[:nb], DET[0], DET[1], w=1.0, out=tmp, accumulate=False)
+    row_ms, col_ms, n_row, n_col = plan.pass_times()
+    plan.set_pass_timing(False)
+    # Algorithmic bytes per launch: one read and one write of the columns the
+    # pass must touch (band skipping leaves out whole propagator columns; the
+    # column pass skips whole tiles of them), plus the shared table (t: every
+    # pixel; P: the kept columns).
+    lo, hi = band
+    skip_row = hi - lo + 1 if hi >= lo else 0
+    tile = tun["col_tile"] or 8
+    skip_col = tile * max(0, (hi + 1) // tile - -(-lo // tile)) if hi >= lo else 0
+    row_bytes = 16 * N_GRID * (N_GRID - skip_row) * probes_per_launch + 8 * N
+    col_bytes = 16 * N_GRID * (N_GRID - skip_col) * probes_per_launch + 8 * N_GRID * (N_GRID - skip_col)
     dom = ("column pass", col_ms, col_bytes) if col_ms >= row_ms else ("row pass", row_ms, row_bytes)
     achieved = dom[2] / (dom[1] * 1e-3)
+    traffic = PMC_BYTES_PER_PROBE[dom[0]] * probes_per_launch
+    copy_bw = copy_bandwidth(torch)
 
     plain_ps = plain_path(em, torch, em.specimen.vibrate(at0, U_RMS, SEED, 1), xs, ys, args.plain_probes)
     t_fwd, t_fwd_adj = adjoint_ratio(em, torch)
@@ -324,9 +361,16 @@ def main():
                    "probe_group": G, "tuning": tun, "autotune_ms_per_probe_slice": tune_ms,
                    "phonon_u_rms": U_RMS, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "launch_ms": dom[1], "algorithmic_bytes_per_launch": dom[2],
-                     "row_pass_ms": row_ms, "col_pass_ms": col_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "launch_ms": dom[1], "launches_timed": n_col if dom[0] == "column pass" else n_row,
+                     "algorithmic_bytes_per_launch": dom[2], "probes_per_launch": probes_per_launch,
+                     "row_pass_ms": row_ms, "col_pass_ms": col_ms,
+                     "row_pass_frac": row_bytes / (row_ms * 1e-3) / HBM_PEAK,
+                     "copy_kernel_GBps": copy_bw / 1e9, "frac_of_copy": achieved / copy_bw,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch (copy-kernel calibrated), "
+                                       "per probe, scaled to probes_per_launch"},
+        # whole slice step against one read + write of the wave per probe-slice
+        # (16 B per pixel) plus the shared potential and propagator tables
         "slice_roofline_frac": ps * (16 * N + 12 * N / G) / HBM_PEAK,
         "band_skip": {"threshold": 0.0 if args.no_band_skip else 1e-9, "skipped_columns": band[1] - band[0] + 1 if band[1] >= band[0] else 0,
                       "of": N_GRID},

@@ -28,7 +28,8 @@ LENS_FIELDS = [
 # Every entry point declared in include/cudaem_amd.h.
 EXPORTS = [
     "em_fft_size_supported", "em_plan_create", "em_plan_destroy", "em_plan_set_stream",
-    "em_plan_constants", "em_plan_set_probe_group", "em_plan_group_size", "em_plan_last_error",
+    "em_plan_constants", "em_plan_set_probe_group", "em_plan_group_size", "em_plan_set_pass_timing",
+    "em_plan_pass_times", "em_plan_last_error",
     "em_set_element_coefs", "em_set_spline_params", "em_plan_set_specimen", "em_slice_info",
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_adjoint",
@@ -79,6 +80,9 @@ _SIGS = {
     "em_plan_constants": ([_P, _PD, _PD, _PD], _I),
     "em_plan_set_probe_group": ([_P, _I], _I),
     "em_plan_group_size": ([_P, _I, ctypes.POINTER(_I)], _I),
+    "em_plan_set_pass_timing": ([_P, _I], _I),
+    "em_plan_pass_times": ([_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "em_plan_last_error": ([_P], ctypes.c_char_p),
     "em_set_element_coefs": ([_P, _I, _I, _PD, _PD, _I, _PD, _PD], _I),
     "em_set_spline_params": ([_P, _I, _D, _D, _D], _I),

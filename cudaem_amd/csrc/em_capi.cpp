@@ -95,7 +95,18 @@ struct em_plan_s {
   int vstore_slices = -1;
   std::map<GraphKey, std::unique_ptr<Graph>> graphs;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
+  // per-launch timing of the STEM slice loop's row / column passes: event
+  // pairs (start, end) of each timed launch, kind 0 = row, 1 = column
+  bool pass_timing = false;
+  std::vector<hipEvent_t> tev;
+  std::vector<int> tkind;
+  void clear_pass_events() {
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+    tev.clear();
+    tkind.clear();
+  }
   ~em_plan_s() {
+    clear_pass_events();
     graphs.clear();
     if (gstream) (void)hipStreamDestroy(gstream);
   }
@@ -232,6 +243,26 @@ em_status col(em_plan p, unsigned mode, const ColArgs& a) {
   return p->hip(launch_col(mode, a, p->stream), "column pass");
 }
 
+// Launch through `run`, bracketed by a pair of timing events when the plan
+// records per-launch pass times (kind 0 = row pass, 1 = column pass).
+template <typename F>
+em_status timed(em_plan p, int kind, F&& run) {
+  if (!p->pass_timing) return run();
+  hipEvent_t e[2];
+  EM_HIP(p, hipEventCreate(&e[0]));
+  if (hipEventCreate(&e[1]) != hipSuccess) {
+    (void)hipEventDestroy(e[0]);
+    return p->fail(EM_ERR_HIP, "event create");
+  }
+  p->tev.push_back(e[0]);
+  p->tev.push_back(e[1]);
+  p->tkind.push_back(kind);
+  EM_HIP(p, hipEventRecord(e[0], p->stream));
+  EM_TRY(run());
+  EM_HIP(p, hipEventRecord(e[1], p->stream));
+  return EM_OK;
+}
+
 em_status ensure_specimen_potential(em_plan p);
 
 // the interior slice step on precomputed transmissions may read t per probe
@@ -363,6 +394,34 @@ em_status em_plan_group_size(em_plan plan, int nprobe, int* g) {
   if (!plan || nprobe <= 0 || !g) return EM_ERR_INVALID;
   *g = group_size(plan, nprobe);
   return EM_OK;
+}
+
+em_status em_plan_set_pass_timing(em_plan plan, int on) {
+  if (!plan) return EM_ERR_INVALID;
+  if (plan->gstream) (void)hipStreamSynchronize(plan->gstream);
+  plan->clear_pass_events();
+  plan->pass_timing = on != 0;
+  return EM_OK;
+}
+
+em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* n_row, int* n_col) {
+  if (!plan || !row_ms || !col_ms || !n_row || !n_col) return EM_ERR_INVALID;
+  double sum[2] = {0, 0};
+  int cnt[2] = {0, 0};
+  em_status st = EM_OK;
+  for (size_t i = 0; i < plan->tkind.size() && st == EM_OK; ++i) {
+    st = plan->hip(hipEventSynchronize(plan->tev[2 * i + 1]), "event sync");
+    float ms = 0;
+    if (st == EM_OK) st = plan->hip(hipEventElapsedTime(&ms, plan->tev[2 * i], plan->tev[2 * i + 1]), "elapsed");
+    sum[plan->tkind[i]] += ms;
+    cnt[plan->tkind[i]] += 1;
+  }
+  plan->clear_pass_events();
+  *row_ms = cnt[0] ? sum[0] / cnt[0] : 0.0;
+  *col_ms = cnt[1] ? sum[1] / cnt[1] : 0.0;
+  *n_row = cnt[0];
+  *n_col = cnt[1];
+  return st;
 }
 
 em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
@@ -794,10 +853,10 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     for (int s = 0; s < S; ++s) {
       RowArgs r = row_args_skip(p, wave, wave, ng, true, s < S - 1);
       r.t = p->Tstore.as<float2>() + (long long)s * N;
-      EM_TRY(row(p, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), r));
+      EM_TRY(timed(p, 0, [&] { return row(p, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), r); }));
       ColArgs cc = col_args_skip(p, wave, wave, ng);
       if (s < S - 1) {
-        EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc));
+        EM_TRY(timed(p, 1, [&] { return col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc); }));
       } else {
         cc.ndet = ndet;
         for (int d = 0; d < ndet; ++d) {
@@ -872,6 +931,14 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
       if ((st = p->hip(hipStreamSynchronize(p->stream), "sync")) != EM_OK) break;
       if ((st = p->hip(hipMemcpy(p->xe.p, xe.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "xe")) != EM_OK) break;
       if ((st = p->hip(hipMemcpy(p->ye.p, ye.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "ye")) != EM_OK) break;
+      if (p->pass_timing) {
+        // timed launches run eagerly: events between the passes need a plain stream
+        if ((st = stem_batch_body(p, nb, S, dg2min, dg2max, ndet)) != EM_OK) break;
+        st = p->hip(launch_det_reduce(nb, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out + start, npos,
+                                      p->stream),
+                    "detector reduce");
+        continue;
+      }
       auto it = graphs.find(nb);
       if (it == graphs.end()) {
         auto g = std::make_unique<Graph>();

@@ -110,6 +110,20 @@ class Plan:
         C.check(C.lib().em_plan_group_size(self._h, int(nprobe), ctypes.byref(g)), self._h)
         return g.value
 
+    def set_pass_timing(self, on):
+        """Record the duration of every interior row / column pass launch of
+        the following stem() calls (they then run without graph capture)."""
+        C.check(C.lib().em_plan_set_pass_timing(self._h, 1 if on else 0), self._h)
+
+    def pass_times(self):
+        """(row ms, column ms, row launches, column launches): average launch
+        durations recorded since set_pass_timing(True); clears the records."""
+        r, c = ctypes.c_double(0), ctypes.c_double(0)
+        nr, nc = ctypes.c_int(0), ctypes.c_int(0)
+        C.check(C.lib().em_plan_pass_times(self._h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(nr),
+                                           ctypes.byref(nc)), self._h)
+        return r.value, c.value, nr.value, nc.value
+
     def tuning(self):
         t = C.em_tuning()
         C.check(C.lib().em_plan_get_tuning(self._h, ctypes.byref(t)), self._h)

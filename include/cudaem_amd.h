@@ -89,6 +89,14 @@ em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
 /* Probes per group the plan uses for a batch of nprobe probes: the target
  * group size, evened out over the batch's groups (the last may be smaller). */
 em_status em_plan_group_size(em_plan plan, int nprobe, int* probes_per_group);
+/* Per-launch timing of em_stem's slice loop (no reference counterpart: it
+ * replaces reading kernel durations off a profiler).  While on, em_stem runs
+ * its batches without graph capture and brackets every interior row pass and
+ * column pass with a pair of HIP events; em_plan_pass_times then returns the
+ * average launch duration of each (ms) and the number of launches averaged,
+ * and clears the records. */
+em_status em_plan_set_pass_timing(em_plan plan, int on);
+em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* n_row, int* n_col);
 /* Launch shape of the fused passes (no reference counterpart: the reference
  * relies on cuFFT's planner).  row_blocks / col_blocks: target workgroups of a
  * row / column pass (the probes each workgroup processes follow from it);

@@ -200,6 +200,34 @@ def test_stem_batches_and_groups_agree(em):
     assert rel_l2(res[2], res[0]) < 1e-6
 
 
+def test_stem_pass_timing_runs_the_same_passes(em):
+    """With per-launch pass timing on, em_stem runs its batches without graph
+    capture and times every interior row pass (all slices) and column pass
+    (all but the last slice) of every probe group; the detector sums equal
+    the graph run's."""
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 4)
+    n = 128
+    xs, ys = sp.raster(5, 5, at["lx"], at["ly"])
+    lens = probe_lens(20.0)
+    with em.Plan(n, n, at["lx"], at["ly"], 1.9525, 300.0, max_batch=10) as p:
+        p.set_probe_group(4)
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+        ref = _np(p.stem(lens, xs, ys, [40.0], [120.0]))
+        p.set_pass_timing(True)
+        got = _np(p.stem(lens, xs, ys, [40.0], [120.0]))
+        row_ms, col_ms, n_row, n_col = p.pass_times()
+        p.set_pass_timing(False)
+        again = _np(p.stem(lens, xs, ys, [40.0], [120.0]))
+        assert p.pass_times()[2:] == (0, 0)
+    # 25 probes in batches of 10, 10, 5; groups of 4 probes -> 3 + 3 + 2 groups
+    ngroups = 8
+    assert n_row == ngroups * S and n_col == ngroups * (S - 1)
+    assert row_ms > 0 and col_ms > 0
+    assert rel_l2(got, ref) < 1e-6
+    assert rel_l2(again, ref) < 1e-6
+
+
 def test_thickness_series_matches_oracle(em):
     import torch
     at, n, dz, e0 = _stem_setup(n=128, cells=3, nz=4)
