@@ -1203,8 +1203,8 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   if (st == EM_OK) st = measure(&cur);
   // coordinate descent over the launch-shape parameters, two sweeps
   const int groups[] = {0, 8, 16, 24, 32, 40, 48, 64, nprobe};
-  const int rowb[] = {1024, 2048, 4096, 8192};
-  const int colb[] = {512, 1024, 2048, 4096, 8192};
+  const int rowb[] = {1024, 2048, 4096, 8192, 16384};
+  const int colb[] = {512, 1024, 2048, 4096, 8192, 16384};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
   const int row_variants[] = {0, 1};
@@ -1216,7 +1216,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
                         : par == 3 ? tiles
                         : par == 4 ? variants
                                    : row_variants;
-      const int nc = par == 0 ? 9 : par == 1 ? 4 : par == 2 ? 5 : 2;
+      const int nc = par == 0 ? 9 : par == 1 ? 5 : par == 2 ? 6 : 2;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;

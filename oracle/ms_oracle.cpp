@@ -98,6 +98,24 @@ void moliere_coefs(int Z, double* a, double* b) {
   }
 }
 
+// The same Moliere screening as a 3D potential in the Kirkland 0-12 form
+// (type 4): V(r) = sum_i e^2/(4 pi eps0) Z alpha_i exp(-beta_i r / a_TF) / r,
+// i.e. cl_i = kEk Z alpha_i [V A], cnl_i = beta_i / a_TF [1/A], no Gaussians.
+// Its projection along z is moliere_coefs' K0 sum: int V dz = 2 cl_i K0(cnl_i R).
+void moliere_3d(int Z, double* cl, double* cnl) {
+  static const double alpha[3] = {0.35, 0.55, 0.10};
+  static const double beta[3] = {0.3, 1.2, 6.0};
+  const double atf = 0.88534 * kA0 * std::pow((double)Z, -1.0 / 3.0);
+  for (int i = 0; i < 3; ++i) {
+    cl[i] = kEk * Z * alpha[i];
+    cnl[i] = beta[i] / atf;
+  }
+  for (int i = 3; i < 6; ++i) {
+    cl[i] = 0.0;
+    cnl[i] = 1.0;
+  }
+}
+
 void vr_kirkland(int nk, const double* a, const double* b, int ng, const double* c,
                  const double* d, double R, double* V, double* dVdR) {
   const double fk = 4.0 * kPi * kPi * kA0 * kEk;
@@ -180,6 +198,163 @@ static inline double spline_eval(double x, const double* R2, const double* c0,
   const int ix = spline_index(x, R2, nR);
   const double dx = x - R2[ix];
   return ((c3[ix] * dx + c2[ix]) * dx + c1[ix]) * dx + c0[ix];
+}
+
+// ---------------------------------------------------------------------------
+// Fine slicing (gen_sa_coeff_kernel + fill_spl_coeff_kernel,
+// device_kernels.cuh:48-105, driven by projected_potential.cuh:73-127).
+//
+// 3D atomic potential V(r) and (1/r) dV/dr of the six parameterisations
+// (Vr_dVrir_*, basic_functions_dev.cuh:104-187), coefficients cl (linear) and
+// cnl (non-linear), both scaled by the quadrature weight f
+// (Vr_dVrir_dev, basic_functions_dev.cuh:189-214):
+//   1 Doyle-Turner 0-4, 2 Peng 0-4: 4 / 5 Gaussians cl e^{-cnl r^2}
+//   3 Peng 0-12:                     5 Gaussians
+//   4 Kirkland 0-12:                 3 Yukawa terms cl e^{-cnl r}/r + 3 Gaussians
+//   5 Weickenmeier-Kohl 0-12:        6 terms cl erfc(cnl r)/r
+//   6 Lobato 0-12:                   5 terms cl (2/(cnl r) + 1) e^{-cnl r}
+// The type numbers are the build's (the reference's enum lives in a missing
+// header); the coefficient tables are supplied by the caller.
+// ---------------------------------------------------------------------------
+void vr3d(int pt, double r, const double* cl, const double* cnl, double f, double* V, double* dVrir) {
+  double v = 0.0, d = 0.0;
+  const double r2 = r * r, ir = 1.0 / r;
+  switch (pt) {
+    case 1: case 2: case 3: {
+      const int n = pt == 1 ? 4 : 5;
+      for (int i = 0; i < n; ++i) {
+        const double e = cl[i] * std::exp(-cnl[i] * r2);
+        v += e;
+        d += -2.0 * cnl[i] * e;
+      }
+      break;
+    }
+    case 4: {
+      for (int i = 0; i < 3; ++i) {
+        const double e = cl[i] * std::exp(-cnl[i] * r) * ir;
+        v += e;
+        d += -e * (cnl[i] + ir) * ir;
+      }
+      for (int i = 3; i < 6; ++i) {
+        const double e = cl[i] * std::exp(-cnl[i] * r2);
+        v += e;
+        d += -2.0 * cnl[i] * e;
+      }
+      break;
+    }
+    case 5: {
+      double g = 0.0;
+      for (int i = 0; i < 6; ++i) {
+        v += cl[i] * std::erfc(cnl[i] * r);
+        g += cl[i] * cnl[i] * std::exp(-cnl[i] * cnl[i] * r2);
+      }
+      v *= ir;
+      d = -(2.0 / std::sqrt(kPi) * g + v) / r2;
+      break;
+    }
+    case 6: {
+      double extra = 0.0;
+      for (int i = 0; i < 5; ++i) {
+        const double br = cnl[i] * r;
+        const double e = cl[i] * std::exp(-br);
+        v += e * (2.0 / br + 1.0);
+        extra += e * (br + 1.0);
+      }
+      d = -(v + extra) / r2;
+      break;
+    }
+    default:
+      break;
+  }
+  *V = v * f;
+  *dVrir = d * f;
+}
+
+// Gauss-Legendre nodes and weights on [-1/2, 1/2] (weights sum to 1), the
+// quadrature gen_sa_coeff_kernel integrates over one slice with
+// (z = dz x + a, weight dz w).  Newton iteration on P_n (textbook).
+void gauss_legendre(int n, double* x, double* w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double t = std::cos(kPi * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) { p1 = t; p0 = 1.0; }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      const double dt = p1 / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-16) break;
+    }
+    {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+    }
+    const double wi = 2.0 / ((1.0 - t * t) * dp * dp);
+    x[i] = -0.5 * t;
+    x[n - 1 - i] = 0.5 * t;
+    w[i] = 0.5 * wi;
+    w[n - 1 - i] = 0.5 * wi;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+// Spline of the potential integrated over one slice of thickness dz whose
+// centre sits at height a above the atom:
+//   c0(R_i) = sum_q dz w_q V(sqrt((dz x_q + a)^2 + R_i^2)),
+//   c1(R_i) = 1/2 sum_q dz w_q (1/r) dV/dr  (= d c0 / d(R^2))
+// (gen_sa_coeff_kernel, device_kernels.cuh:48-94), then the R^2 taper of
+// apply_tapering_dev and the cubic coefficients of get_cubic_poly_coef
+// (fill_spl_coeff_kernel, device_kernels.cuh:95-105).  Nodes as build_spline.
+void fine_spline(int pt, const double* cl, const double* cnl, double a, double dz, int nq,
+                 double r_min, double r_max, double tap_frac, int nR, double* R2, double* c0,
+                 double* c1, double* c2, double* c3) {
+  std::vector<double> qx(nq), qw(nq);
+  gauss_legendre(nq, qx.data(), qw.data());
+  const double r_tap = tap_frac * r_max;
+  const double r2_tap = r_tap * r_tap, r2_max = r_max * r_max;
+  const double tap_cf = 0.5 * kPi / (r2_max - r2_tap);
+  const double dl = std::log(r_max / r_min) / (nR - 1);
+  for (int i = 0; i < nR; ++i) {
+    const double r = (i == nR - 1) ? r_max : r_min * std::exp(i * dl);
+    const double x = r * r;
+    double v = 0.0, dv = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      const double z = dz * qx[q] + a;
+      double vq, dq;
+      vr3d(pt, std::sqrt(z * z + x), cl, cnl, dz * qw[q], &vq, &dq);
+      v += vq;
+      dv += dq;
+    }
+    dv *= 0.5;
+    if (r2_tap < x) {
+      const double tap = std::cos(tap_cf * (x - r2_tap));
+      const double dtap = std::sin(tap_cf * (x - r2_tap));
+      dv = dv * tap - tap_cf * v * dtap;
+      v *= tap;
+    }
+    R2[i] = x;
+    c0[i] = v;
+    c1[i] = dv;
+  }
+  for (int i = 0; i < nR - 1; ++i) {
+    const double idR = 1.0 / (R2[i + 1] - R2[i]);
+    const double m = (c0[i + 1] - c0[i]) * idR;
+    const double n = c1[i] + c1[i + 1];
+    c2[i] = (3.0 * m - n - c1[i]) * idR;
+    c3[i] = (n - 2.0 * m) * idR * idR;
+  }
+  c2[nR - 1] = 0.0;
+  c3[nR - 1] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -429,6 +604,45 @@ int slicing(int n, const double* z, double dz, int border, double eps, int max_s
   return nv - 1;
 }
 
+// Fine-sliced mode of the same generator (slice_generator.hpp:88-93,
+// 108-128): bounds cover [z_min - R_int, z_max + R_int] (the integration range
+// of Atom_Coord_Data's statistics, in a missing header: the build takes
+// R_int = the largest spline radius), and slice k takes the atoms with z in
+// [min(z_mid - R_int, z_start), max(z_mid + R_int, z_end)] widened by eps on
+// both sides (Inc_Borders).
+int slicing_fine(int n, const double* z, double dz, int border, double eps, double r_int,
+                 int max_slices, double* bounds, long long* istart, long long* iend) {
+  if (n <= 0) return 0;
+  const double vmin = z[0] - r_int, vmax = z[n - 1] + r_int;
+  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
+  if (nv - 1 > max_slices) return -(nv - 1);
+  std::vector<double> b(nv);
+  if (border == 0) {
+    for (int i = 0; i < nv; ++i) b[i] = (vmin - 2 * eps) + i * dz;
+  } else if (border == 1) {
+    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = (vmax + 2 * eps) - i * dz;
+  } else {
+    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
+    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
+  }
+  for (int i = 0; i < nv; ++i) bounds[i] = b[i];
+  for (int s = 0; s < nv - 1; ++s) {
+    const double zm = 0.5 * (b[s] + b[s + 1]);
+    const double zs = std::fmin(zm - r_int, b[s]) - eps, ze = std::fmax(zm + r_int, b[s + 1]) + eps;
+    long long i0, i1;
+    if (zs > ze || z[n - 1] < zs || ze < z[0]) {
+      i0 = 1; i1 = 0;
+    } else {
+      i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z, z + n, zs) - z);
+      i1 = (z[n - 1] <= ze) ? (n - 1) : (long long)(std::lower_bound(z, z + n, ze) - z) - 1;
+      if (i0 > i1 || z[i1] < zs || ze < z[i0]) { i0 = 1; i1 = 0; }
+    }
+    istart[s] = i0;
+    iend[s] = i1;
+  }
+  return nv - 1;
+}
+
 // ---------------------------------------------------------------------------
 // One slice of projected potential, gen_slice_potential_kernel
 // (device_kernels.cuh:28-47) over all atoms of the slice: V += coeff*occ*S(R^2)
@@ -457,6 +671,38 @@ void potential(const Grid& g, long long i0, long long i1, const double* x, const
         const double rr = ddx * ddx + ddy * ddy;
         if (rr < rm2)
           V[(long long)iy * g.nx + ix] += coeff * occ[a] * spline_eval(rr, r2, a0, a1, a2, a3, nR);
+      }
+  }
+}
+
+// One fine-sliced slice [zs, ze]: every atom of its range contributes the
+// spline of the potential integrated over the slice at its own height
+// (projected_potential.cuh:108-116: z_middle = slice centre - atom z).
+// Element e of atom a uses pt[e], cl/cnl[e*6 ..], rmax2[e]; splines on nR nodes.
+void potential_fine(const Grid& g, double zs, double ze, long long i0, long long i1,
+                    const double* x, const double* y, const double* z, const double* occ,
+                    const int* type, const int* pt, const double* cl, const double* cnl, int nq,
+                    double r_min, double r_max, double tap_frac, int nR, double coeff, double* V) {
+  const long long N = g.n();
+  for (long long i = 0; i < N; ++i) V[i] = 0.0;
+  std::vector<double> R2(nR), c0(nR), c1(nR), c2(nR), c3(nR);
+  const double dz = ze - zs, zm = 0.5 * (zs + ze);
+  const double rm2 = r_max * r_max, rm = r_max;
+  for (long long a = i0; a <= i1; ++a) {
+    const int t = type[a];
+    fine_spline(pt[t], cl + 6 * t, cnl + 6 * t, zm - z[a], dz, nq, r_min, r_max, tap_frac, nR,
+                R2.data(), c0.data(), c1.data(), c2.data(), c3.data());
+    int x0 = std::max(0, (int)std::floor((x[a] - rm) / g.dx) - 1);
+    int x1 = std::min(g.nx - 1, (int)std::ceil((x[a] + rm) / g.dx) + 1);
+    int y0 = std::max(0, (int)std::floor((y[a] - rm) / g.dy) - 1);
+    int y1 = std::min(g.ny - 1, (int)std::ceil((y[a] + rm) / g.dy) + 1);
+    for (int iy = y0; iy <= y1; ++iy)
+      for (int ix = x0; ix <= x1; ++ix) {
+        const double ddx = ix * g.dx - x[a], ddy = iy * g.dy - y[a];
+        const double rr = ddx * ddx + ddy * ddy;
+        if (rr < rm2)
+          V[(long long)iy * g.nx + ix] +=
+              coeff * occ[a] * spline_eval(rr, R2.data(), c0.data(), c1.data(), c2.data(), c3.data(), nR);
       }
   }
 }
@@ -551,6 +797,36 @@ void orc_fft2d(double* data, int nx, int ny, int sign) {
 int orc_slicing(int n, const double* z, double dz, int border, double eps, int max_slices,
                 double* bounds, long long* istart, long long* iend) {
   return slicing(n, z, dz, border, eps, max_slices, bounds, istart, iend);
+}
+
+int orc_slicing_fine(int n, const double* z, double dz, int border, double eps, double r_int,
+                     int max_slices, double* bounds, long long* istart, long long* iend) {
+  return slicing_fine(n, z, dz, border, eps, r_int, max_slices, bounds, istart, iend);
+}
+
+void orc_vr3d(int pt, int n, const double* r, const double* cl, const double* cnl, double* V,
+              double* dVrir) {
+  for (int i = 0; i < n; ++i) vr3d(pt, r[i], cl, cnl, 1.0, V + i, dVrir + i);
+}
+
+void orc_gauss_legendre(int n, double* x, double* w) { gauss_legendre(n, x, w); }
+
+void orc_moliere_3d(int Z, double* cl, double* cnl) { moliere_3d(Z, cl, cnl); }
+
+void orc_fine_spline(int pt, const double* cl, const double* cnl, double a, double dz, int nq,
+                     double r_min, double r_max, double tap_frac, int nR, double* R2, double* c0,
+                     double* c1, double* c2, double* c3) {
+  fine_spline(pt, cl, cnl, a, dz, nq, r_min, r_max, tap_frac, nR, R2, c0, c1, c2, c3);
+}
+
+void orc_potential_fine(int nx, int ny, double lx, double ly, double zs, double ze, long long i0,
+                        long long i1, const double* x, const double* y, const double* z,
+                        const double* occ, const int* type, const int* pt, const double* cl,
+                        const double* cnl, int nq, double r_min, double r_max, double tap_frac,
+                        int nR, double coeff, double* V) {
+  Grid g(nx, ny, lx, ly, 0.0, 0);
+  potential_fine(g, zs, ze, i0, i1, x, y, z, occ, type, pt, cl, cnl, nq, r_min, r_max, tap_frac, nR,
+                 coeff, V);
 }
 
 void orc_potential(int nx, int ny, double lx, double ly, long long i0, long long i1,

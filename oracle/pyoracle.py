@@ -70,6 +70,14 @@ def lib():
                                _I, _PD, _PD, _D, _PD, _I]
         L.orc_stem.restype = _I
         L.orc_adjoint.argtypes = [_I, _I, _PD, _I, _PD, _I, _PD, _PD, _PD, _PD]
+        L.orc_slicing_fine.argtypes = [_I, _PD, _D, _I, _D, _D, _I, _PD, _PLL, _PLL]
+        L.orc_slicing_fine.restype = _I
+        L.orc_vr3d.argtypes = [_I, _I, _PD, _PD, _PD, _PD, _PD]
+        L.orc_gauss_legendre.argtypes = [_I, _PD, _PD]
+        L.orc_moliere_3d.argtypes = [_I, _PD, _PD]
+        L.orc_fine_spline.argtypes = [_I, _PD, _PD, _D, _D, _I, _D, _D, _D, _I, _PD, _PD, _PD, _PD, _PD]
+        L.orc_potential_fine.argtypes = [_I, _I, _D, _D, _D, _D, _LL, _LL, _PD, _PD, _PD, _PD, _PI,
+                                         _PI, _PD, _PD, _I, _D, _D, _D, _I, _D, _PD]
         _lib = L
     return _lib
 
@@ -199,6 +207,78 @@ def potential(nx, ny, lx, ly, i0, i1, x, y, occ, types, tables, coeff):
     lib().orc_potential(nx, ny, lx, ly, int(i0), int(i1), _pd(x), _pd(y), _pd(occ),
                         types.ctypes.data_as(_PI), t.nR, _pd(t.R2), _pd(t.c0), _pd(t.c1),
                         _pd(t.c2), _pd(t.c3), _pd(t.rmax2), coeff, _pd(V))
+    return V
+
+
+# --------------------------------------------------------------------------- fine slicing
+# Parameterisation types of the 3D atomic potential (the build's numbering).
+PT_DOYLE_0_4, PT_PENG_0_4, PT_PENG_0_12, PT_KIRKLAND_0_12, PT_WEICKENMEIER_0_12, PT_LOBATO_0_12 = 1, 2, 3, 4, 5, 6
+# Gauss-Legendre points per slice of the z integration (the product uses the same).
+FINE_NQ = 128
+
+
+def _coef6(c):
+    out = np.zeros(6)
+    c = np.asarray(c, dtype=np.float64).ravel()
+    out[:len(c)] = c
+    return out
+
+
+def vr3d(pt, r, cl, cnl):
+    r = _f64(r)
+    cl, cnl = _coef6(cl), _coef6(cnl)
+    V = np.zeros_like(r)
+    d = np.zeros_like(r)
+    lib().orc_vr3d(pt, len(r), _pd(r), _pd(cl), _pd(cnl), _pd(V), _pd(d))
+    return V, d
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauss_legendre(n, _pd(x), _pd(w))
+    return x, w
+
+
+def moliere_3d(Z):
+    cl = np.zeros(6)
+    cnl = np.zeros(6)
+    lib().orc_moliere_3d(int(Z), _pd(cl), _pd(cnl))
+    return cl, cnl
+
+
+def fine_spline(pt, cl, cnl, a, dz, nq=FINE_NQ, r_min=SPLINE_RMIN, r_max=SPLINE_RMAX, tap=SPLINE_TAP,
+                nR=SPLINE_NR):
+    cl, cnl = _coef6(cl), _coef6(cnl)
+    out = [np.zeros(nR) for _ in range(5)]
+    lib().orc_fine_spline(pt, _pd(cl), _pd(cnl), a, dz, nq, r_min, r_max, tap, nR, *[_pd(o) for o in out])
+    return out  # R2, c0, c1, c2, c3
+
+
+def slicing_fine(z_sorted, dz, border, r_int, eps=Z_EPS, max_slices=100000):
+    z = _f64(z_sorted)
+    bounds = np.zeros(max_slices + 1)
+    i0 = np.zeros(max_slices, dtype=np.int64)
+    i1 = np.zeros(max_slices, dtype=np.int64)
+    S = lib().orc_slicing_fine(len(z), _pd(z), dz, border, eps, r_int, max_slices, _pd(bounds),
+                               i0.ctypes.data_as(_PLL), i1.ctypes.data_as(_PLL))
+    if S < 0:
+        raise ValueError("too many slices")
+    return bounds[:S + 1].copy(), i0[:S].copy(), i1[:S].copy()
+
+
+def potential_fine(nx, ny, lx, ly, zs, ze, i0, i1, x, y, z, occ, types, pts, cls, cnls, coeff,
+                   nq=FINE_NQ, r_min=SPLINE_RMIN, r_max=SPLINE_RMAX, tap=SPLINE_TAP, nR=SPLINE_NR):
+    """One fine-sliced slice; element e uses pts[e], cls[e], cnls[e]."""
+    x, y, z, occ = _f64(x), _f64(y), _f64(z), _f64(occ)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    pts = np.ascontiguousarray(pts, dtype=np.int32)
+    cl = np.ascontiguousarray(np.stack([_coef6(c) for c in cls]))
+    cnl = np.ascontiguousarray(np.stack([_coef6(c) for c in cnls]))
+    V = np.zeros((ny, nx))
+    lib().orc_potential_fine(nx, ny, lx, ly, zs, ze, int(i0), int(i1), _pd(x), _pd(y), _pd(z), _pd(occ),
+                             types.ctypes.data_as(_PI), pts.ctypes.data_as(_PI), _pd(cl), _pd(cnl), nq,
+                             r_min, r_max, tap, nR, coeff, _pd(V))
     return V
 
 

@@ -40,6 +40,41 @@ def moliere_projected(Z, R):
     return 2 * v
 
 
+# 3D potential parameterisations (type numbers of the build), a seeded-free
+# fixed coefficient set per type, decaying fast enough that |z| <= 6 A holds
+# the whole integral to 1e-10.
+FINE_SETS = {
+    1: ([2.0, 5.0, 10.0, 20.0], [1.0, 2.0, 8.0, 30.0]),
+    2: ([1.0, 4.0, 9.0, 15.0, 3.0], [1.2, 3.0, 7.0, 25.0, 60.0]),
+    3: ([0.5, 2.0, 6.0, 12.0, 8.0], [1.1, 2.5, 9.0, 30.0, 90.0]),
+    4: ([30.0, 60.0, 10.0, 3.0, 6.0, 1.0], [4.0, 9.0, 20.0, 2.0, 10.0, 40.0]),
+    5: ([1.0, 3.0, 5.0, 10.0, 20.0, 5.0], [0.8, 1.5, 3.0, 6.0, 12.0, 25.0]),
+    6: ([1.0, 3.0, 8.0, 15.0, 4.0], [4.0, 6.0, 10.0, 20.0, 40.0]),
+}
+
+
+def vr3d_published(pt, r, cl, cnl):
+    """V(r) of the published parameterisations (Doyle-Turner, Peng, Kirkland,
+    Weickenmeier-Kohl, Lobato), written from their definitions."""
+    if pt in (1, 2, 3):
+        return sum(c * np.exp(-b * r * r) for c, b in zip(cl, cnl))
+    if pt == 4:
+        return (sum(c * np.exp(-b * r) / r for c, b in zip(cl[:3], cnl[:3]))
+                + sum(c * np.exp(-b * r * r) for c, b in zip(cl[3:], cnl[3:])))
+    if pt == 5:
+        return sum(c * ss.erfc(b * r) for c, b in zip(cl, cnl)) / r
+    return sum(c * (2.0 / (b * r) + 1.0) * np.exp(-b * r) for c, b in zip(cl, cnl))
+
+
+def slab_integral(pt, R, a, dz):
+    cl, cnl = FINE_SETS[pt]
+    f = lambda z: vr3d_published(pt, np.sqrt(z * z + R * R), cl, cnl)
+    lo, hi = a - dz / 2, a + dz / 2
+    pts = [0.0] if lo < 0 < hi else None
+    v, _ = si.quad(f, lo, hi, points=pts, epsabs=0, epsrel=1e-13, limit=500)
+    return v
+
+
 def main():
     out = {}
     kv = np.array([80.0, 200.0, 300.0])
@@ -62,6 +97,20 @@ def main():
     out["vr_R"] = Rs
     for Z in (8, 22, 38):
         out[f"vr_{Z}"] = np.array([moliere_projected(Z, R) for R in Rs])
+    for n in (7, 64, 128):
+        x, w = np.polynomial.legendre.leggauss(n)
+        out[f"gl_x_{n}"] = x / 2
+        out[f"gl_w_{n}"] = w / 2
+    # spline nodes of the build (128 nodes, log-spaced in R over [0.005, 3] A):
+    # every 8th node from 0.1 A on
+    nodes = 0.005 * np.exp(np.arange(128) * np.log(3.0 / 0.005) / 127)
+    fine_R = nodes[(nodes > 0.1) & (np.arange(128) % 8 == 0)]
+    out["fine_R"] = fine_R
+    for pt, (cl, cnl) in FINE_SETS.items():
+        out[f"fine_cl_{pt}"] = np.array(cl)
+        out[f"fine_cnl_{pt}"] = np.array(cnl)
+        # one 1.9525 A slice centred 0.7 A above the atom
+        out[f"fine_slab_{pt}"] = np.array([slab_integral(pt, R, 0.7, 1.9525) for R in fine_R])
     np.savez(os.path.join(HERE, "golden.npz"), **out)
     print("wrote", os.path.join(HERE, "golden.npz"))
 

@@ -354,3 +354,179 @@ def test_adjoint_dot_product():
     lhs = np.vdot(y[0], Gx)
     rhs = np.vdot(GHy, x[0])
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+# ----------------------------------------------------------------- fine slicing
+FINE_TYPES = [1, 2, 3, 4, 5, 6]
+
+
+def _fine_set(pt):
+    return GOLD[f"fine_cl_{pt}"], GOLD[f"fine_cnl_{pt}"]
+
+
+@pytest.mark.parametrize("n", [7, 64, 128])
+def test_gauss_legendre_matches_numpy(n):
+    x, w = O.gauss_legendre(n)
+    np.testing.assert_allclose(x, GOLD[f"gl_x_{n}"], rtol=0, atol=2e-15)
+    np.testing.assert_allclose(w, GOLD[f"gl_w_{n}"], rtol=1e-10, atol=1e-15)
+    # exact for polynomials of degree <= 2n - 1 on [-1/2, 1/2]
+    for k in range(0, 2 * n, max(1, n // 4)):
+        exact = 0.0 if k % 2 else 2 * 0.5 ** (k + 1) / (k + 1)
+        assert abs(np.dot(w, x ** k) - exact) < 1e-14
+
+
+@pytest.mark.parametrize("pt", FINE_TYPES)
+def test_vr3d_derivative_by_finite_differences(pt):
+    """dVrir = (1/r) dV/dr of every parameterisation."""
+    cl, cnl = _fine_set(pt)
+    r = np.linspace(0.15, 2.8, 23)
+    h = 1e-6
+    V, d = O.vr3d(pt, r, cl, cnl)
+    Vp, _ = O.vr3d(pt, r + h, cl, cnl)
+    Vm, _ = O.vr3d(pt, r - h, cl, cnl)
+    fd = (Vp - Vm) / (2 * h) / r
+    np.testing.assert_allclose(d, fd, rtol=2e-7, atol=1e-9 * np.abs(V).max())
+
+
+def _projection_closed_form(pt, R):
+    """int V(sqrt(z^2 + R^2)) dz over all z: Gaussians -> c sqrt(pi/b) e^{-b R^2};
+    Yukawa c e^{-b r}/r -> 2 c K0(b R); Weickenmeier c erfc(b r)/r -> c E1(b^2 R^2);
+    Lobato c (2/(b r) + 1) e^{-b r} -> c (4/b K0(b R) + 2 R K1(b R))."""
+    from scipy.special import exp1, k0, k1
+    cl, cnl = _fine_set(pt)
+    if pt in (1, 2, 3):
+        return sum(c * np.sqrt(np.pi / b) * np.exp(-b * R * R) for c, b in zip(cl, cnl))
+    if pt == 4:
+        return (sum(2 * c * k0(b * R) for c, b in zip(cl[:3], cnl[:3]))
+                + sum(c * np.sqrt(np.pi / b) * np.exp(-b * R * R) for c, b in zip(cl[3:], cnl[3:])))
+    if pt == 5:
+        return sum(c * exp1(b * b * R * R) for c, b in zip(cl, cnl))
+    return sum(c * (4 / b * k0(b * R) + 2 * R * k1(b * R)) for c, b in zip(cl, cnl))
+
+
+@pytest.mark.parametrize("pt", FINE_TYPES)
+def test_fine_spline_thick_slab_is_the_projection(pt):
+    """A 12 A slab around the atom integrates the whole potential: the node
+    values equal the closed-form projections; c1 = d c0 / d(R^2)."""
+    cl, cnl = _fine_set(pt)
+    R2, c0, c1, c2, c3 = O.fine_spline(pt, cl, cnl, 0.0, 12.0, nq=1024, tap=0.999)
+    R = np.sqrt(R2)
+    m = (R > 0.25) & (R < 2.9)
+    np.testing.assert_allclose(c0[m], _projection_closed_form(pt, R[m]), rtol=1e-9, atol=1e-10 * abs(c0).max())
+    h = 1e-5
+    fd = (_projection_closed_form(pt, np.sqrt(R2[m] + h)) - _projection_closed_form(pt, np.sqrt(R2[m] - h))) / (2 * h)
+    np.testing.assert_allclose(c1[m], fd, rtol=1e-6, atol=1e-9 * np.abs(c0).max())
+
+
+@pytest.mark.parametrize("pt", FINE_TYPES)
+def test_fine_spline_one_slice_matches_quadrature_golden(pt):
+    """One 1.9525 A slice centred 0.7 A above the atom (the per-slice
+    quantity of fine slicing): node values against adaptive quadrature of the
+    published forms at the node radii."""
+    cl, cnl = _fine_set(pt)
+    R2, c0 = O.fine_spline(pt, cl, cnl, 0.7, 1.9525, tap=0.999)[:2]
+    R = GOLD["fine_R"]
+    idx = np.searchsorted(R2, R * R * (1 - 1e-12))
+    np.testing.assert_allclose(R2[idx], R * R, rtol=1e-12)
+    np.testing.assert_allclose(c0[idx], GOLD[f"fine_slab_{pt}"], rtol=1e-10, atol=1e-12 * abs(c0).max())
+
+
+def test_fine_slices_sum_to_one_slab_and_moliere_3d_projects_to_moliere():
+    """Splitting a slab into slices and summing is exact up to quadrature
+    round-off; the Moliere 3D form over a slab holding the whole potential
+    reproduces the K0 projected form the direct path uses."""
+    cl, cnl = O.moliere_3d(38)
+    dz = 1.9525
+    L = 15 * dz
+    parts = [O.fine_spline(O.PT_KIRKLAND_0_12, cl, cnl, -L / 2 + (k + 0.5) * dz, dz, tap=0.999)
+             for k in range(15)]
+    whole = O.fine_spline(O.PT_KIRKLAND_0_12, cl, cnl, 0.0, L, nq=2048, tap=0.999)
+    R = np.sqrt(whole[0])
+    m = (R > 0.2) & (R < 2.5)
+    s = sum(p[1] for p in parts)
+    np.testing.assert_allclose(s[m], whole[1][m], rtol=1e-9)
+    proj = O.build_spline(38, tap=0.999)
+    np.testing.assert_allclose(whole[1][m], proj[1][m], rtol=1e-6)
+
+
+def test_fine_spline_taper_and_cubic():
+    """Tapered nodes vanish at R_max; each cubic reproduces the node values
+    and slopes at both ends of its interval."""
+    cl, cnl = O.moliere_3d(22)
+    R2, c0, c1, c2, c3 = O.fine_spline(O.PT_KIRKLAND_0_12, cl, cnl, 0.3, 1.9525)
+    assert abs(c0[-1]) < 1e-12 * abs(c0).max()
+    d = np.diff(R2)
+    end = ((c3[:-1] * d + c2[:-1]) * d + c1[:-1]) * d + c0[:-1]
+    slope = (3 * c3[:-1] * d + 2 * c2[:-1]) * d + c1[:-1]
+    np.testing.assert_allclose(end, c0[1:], rtol=1e-10, atol=1e-12 * abs(c0).max())
+    np.testing.assert_allclose(slope, c1[1:], rtol=1e-8, atol=1e-10 * abs(c1).max())
+
+
+@pytest.mark.parametrize("border", [O.BORDER_MIN, O.BORDER_MAX, O.BORDER_MIDDLE])
+def test_slicing_fine_membership_brute_force(border):
+    rng = np.random.default_rng(7 + border)
+    z = np.sort(rng.uniform(0, 20, 300))
+    dz, r_int, eps = 1.35775, 3.0, O.Z_EPS
+    b, i0, i1 = O.slicing_fine(z, dz, border, r_int)
+    assert b[0] <= z[0] - r_int + 3 * eps and b[-1] >= z[-1] + r_int - 3 * eps
+    np.testing.assert_allclose(np.diff(b), dz, rtol=1e-12)
+    for s in range(len(i0)):
+        zm = 0.5 * (b[s] + b[s + 1])
+        lo, hi = min(zm - r_int, b[s]) - eps, max(zm + r_int, b[s + 1]) + eps
+        idx = np.nonzero((z >= lo) & (z < hi))[0]
+        if len(idx) == 0:
+            assert i0[s] > i1[s]
+        else:
+            assert (i0[s], i1[s]) == (idx[0], idx[-1])
+
+
+def test_potential_fine_brute_force_and_slice_sum():
+    """Each fine slice equals sigma-free sum over atoms of the potential
+    integrated over the slice (adaptive quadrature at a few pixels); summed
+    over the slices holding an atom it equals that atom's slab over their
+    union."""
+    from scipy.integrate import quad
+    nx = ny = 24
+    lx = ly = 6.0
+    x = np.array([2.1, 3.9, 3.0])
+    y = np.array([2.5, 3.2, 4.1])
+    z = np.array([0.4, 0.9, 1.6])
+    occ = np.array([1.0, 0.7, 1.0])
+    types = np.array([0, 1, 0])
+    pts = [O.PT_KIRKLAND_0_12, O.PT_LOBATO_0_12]
+    cls = [O.moliere_3d(14)[0], GOLD["fine_cl_6"]]
+    cnls = [O.moliere_3d(14)[1], GOLD["fine_cnl_6"]]
+    b, i0, i1 = O.slicing_fine(z, 1.35775, O.BORDER_MIN, 3.0)
+    S = len(i0)
+    Vs = [O.potential_fine(nx, ny, lx, ly, b[s], b[s + 1], i0[s], i1[s], x, y, z, occ, types, pts, cls,
+                           cnls, 1.0) for s in range(S)]
+    dx = lx / nx
+    checked = 0
+    for s in (1, S // 2, S - 2):
+        for iy in range(0, ny, 3):
+            for ix in range(0, nx, 3):
+                Rs = [np.hypot(ix * dx - x[a], iy * dx - y[a]) for a in range(i0[s], i1[s] + 1)]
+                # pixels whose atoms are all off the core and off the taper zone,
+                # including the spline interval that straddles its start
+                if any(0.8 * O.SPLINE_RMAX <= R < O.SPLINE_RMAX or R < 0.2 for R in Rs):
+                    continue
+                tot = 0.0
+                for a, R in zip(range(i0[s], i1[s] + 1), Rs):
+                    if R >= O.SPLINE_RMAX:
+                        continue
+                    cl, cnl = O._coef6(cls[types[a]]), O._coef6(cnls[types[a]])
+                    f = lambda zz: O.vr3d(pts[types[a]], np.array([np.hypot(zz, R)]), cl, cnl)[0][0]
+                    tot += occ[a] * quad(f, b[s] - z[a], b[s + 1] - z[a], epsrel=1e-12)[0]
+                # cubic interpolation between nodes: up to a few 1e-5 relative
+                assert abs(Vs[s][iy, ix] - tot) <= 5e-5 * max(1.0, abs(tot))
+                checked += 1
+    assert checked > 20
+    total = np.sum(Vs, axis=0)
+    ref = np.zeros((ny, nx))
+    for a in range(3):
+        ks = [s for s in range(S) if i0[s] <= a <= i1[s]]
+        assert ks == list(range(ks[0], ks[-1] + 1))
+        zs, ze = b[ks[0]], b[ks[-1] + 1]
+        ref += O.potential_fine(nx, ny, lx, ly, zs, ze, a, a, x, y, z, occ, types, pts, cls, cnls, 1.0,
+                                nq=2048)
+    np.testing.assert_allclose(total, ref, rtol=0, atol=1e-8 * abs(ref).max())
