@@ -30,11 +30,12 @@ EXPORTS = [
     "em_fft_size_supported", "em_plan_create", "em_plan_destroy", "em_plan_set_stream",
     "em_plan_constants", "em_plan_set_probe_group", "em_plan_group_size", "em_plan_set_pass_timing",
     "em_plan_pass_times", "em_plan_last_error",
-    "em_set_element_coefs", "em_set_spline_params", "em_plan_set_specimen", "em_slice_info",
+    "em_set_element_coefs", "em_set_spline_params", "em_set_element_params", "em_plan_set_fine_slicing",
+    "em_plan_set_specimen", "em_slice_info",
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_adjoint",
     "em_detector_sums", "em_ctf_image", "em_diffraction", "em_stem", "em_thickness",
-    "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_spline",
+    "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
     "em_plan_set_band_skip",
 ]
@@ -66,6 +67,9 @@ class em_atoms(ctypes.Structure):
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
+
+# 3D potential parameterisations (EM_PT_* of include/cudaem_amd.h)
+PT_DOYLE_0_4, PT_PENG_0_4, PT_PENG_0_12, PT_KIRKLAND_0_12, PT_WEICKENMEIER_0_12, PT_LOBATO_0_12 = 1, 2, 3, 4, 5, 6
 _D = ctypes.c_double
 _F = ctypes.c_float
 _PD = ctypes.POINTER(ctypes.c_double)
@@ -86,6 +90,8 @@ _SIGS = {
     "em_plan_last_error": ([_P], ctypes.c_char_p),
     "em_set_element_coefs": ([_P, _I, _I, _PD, _PD, _I, _PD, _PD], _I),
     "em_set_spline_params": ([_P, _I, _D, _D, _D], _I),
+    "em_set_element_params": ([_P, _I, _I, _PD, _PD], _I),
+    "em_plan_set_fine_slicing": ([_P, _I, _I], _I),
     "em_plan_set_specimen": ([_P, ctypes.POINTER(em_atoms), _I, _PI], _I),
     "em_slice_info": ([_P, _I, _PD, _PD, _PLL, _PLL], _I),
     "em_potential_slices": ([_P, _I, _I, _P], _I),
@@ -105,6 +111,7 @@ _SIGS = {
                               _I, _PD, _PD, _I, ctypes.POINTER(_P), _P], _I),
     "em_time_passes": ([_P, _P, _P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
     "em_host_slicing": ([ctypes.c_longlong, _PD, _D, _I, _I, _PI, _PD, _PLL, _PLL], _I),
+    "em_host_slicing_fine": ([ctypes.c_longlong, _PD, _D, _I, _D, _I, _PI, _PD, _PLL, _PLL], _I),
     "em_host_spline": ([_I, _I, _D, _D, _D, _PD, _PD, _PD, _PD, _PD], _I),
     "em_host_constants": ([_D, _PD, _PD, _PD], _I),
     "em_plan_get_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),

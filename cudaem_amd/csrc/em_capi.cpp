@@ -80,6 +80,16 @@ struct em_plan_s {
   int nR = 128;
   double r_min = 0.005, r_max = 3.0, tap = 0.85;
   std::map<int, ElementCoefs> custom;
+  // fine slicing (projected_potential.cuh:73-91): a spline per atom entry of
+  // every slice, from the element's 3D potential integrated over the slice
+  struct Params3D {
+    int pt;
+    double cl[6], cnl[6];
+  };
+  bool fine = false;
+  int nqz = 128;
+  std::map<int, Params3D> params3d;
+  long long fine_entries = 0;  // atom entries of the current specimen's splines
   // specimen (host)
   bool has_specimen = false;
   std::vector<double> ax, ay, az, aocc;
@@ -92,6 +102,7 @@ struct em_plan_s {
   // device
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
+  DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
   int vstore_slices = -1;
   std::map<GraphKey, std::unique_ptr<Graph>> graphs;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
@@ -470,6 +481,29 @@ em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max,
   return EM_OK;
 }
 
+em_status em_set_element_params(em_plan plan, int Z, int ptype, const double* cl, const double* cnl) {
+  if (!plan || Z <= 0 || Z > 103 || ptype < EM_PT_DOYLE_0_4 || ptype > EM_PT_LOBATO_0_12 || !cl || !cnl)
+    return EM_ERR_INVALID;
+  em_plan_s::Params3D q;
+  q.pt = ptype;
+  const int nterms = (ptype == EM_PT_DOYLE_0_4) ? 4 : (ptype == EM_PT_KIRKLAND_0_12 || ptype == EM_PT_WEICKENMEIER_0_12) ? 6 : 5;
+  for (int i = 0; i < 6; ++i) {
+    q.cl[i] = i < nterms ? cl[i] : 0.0;
+    q.cnl[i] = i < nterms ? cnl[i] : 1.0;
+    // the forms divide by, or exponentiate with, the non-linear coefficients
+    if (i < nterms && !(cnl[i] > 0)) return EM_ERR_INVALID;
+  }
+  plan->params3d[Z] = q;
+  return EM_OK;
+}
+
+em_status em_plan_set_fine_slicing(em_plan plan, int enable, int nq) {
+  if (!plan || nq < 1 || nq > kMaxFineQ) return EM_ERR_INVALID;
+  plan->fine = enable != 0;
+  plan->nqz = nq;
+  return EM_OK;
+}
+
 em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n_slices) {
   if (!p || !at || at->n < 0 || (at->n > 0 && (!at->x || !at->y || !at->z || !at->Z))) return EM_ERR_INVALID;
   if (border < 0 || border > 2) return EM_ERR_INVALID;
@@ -498,21 +532,24 @@ em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n
     const ElementCoefs k = it != p->custom.end() ? it->second : moliere_coefs(Z);
     p->splines.push_back(build_spline(k, p->nR, p->r_min, p->r_max, p->tap));
   }
-  p->slices = make_slices(p->az, p->grid.dz, border, kZEps);
-  const int S = (int)p->slices.size();
-  // bins of size >= R_max covering [-bin, lx + bin) x [-bin, ly + bin)
   double rmax = 0;
   for (auto& s : p->splines) rmax = std::max(rmax, std::sqrt(s.rmax2));
+  p->slices = p->fine ? make_slices_fine(p->az, p->grid.dz, border, kZEps, rmax)
+                      : make_slices(p->az, p->grid.dz, border, kZEps);
+  const int S = (int)p->slices.size();
+  // bins of size >= R_max covering [-bin, lx + bin) x [-bin, ly + bin)
   const double bin = std::max(rmax, 1e-3);
   p->bin = (float)bin;
   p->nbx = (int)std::ceil((p->grid.lx + 2 * bin) / bin);
   p->nby = (int)std::ceil((p->grid.ly + 2 * bin) / bin);
   const int nb = p->nbx * p->nby;
   std::vector<float4> atoms_h;
+  std::vector<float> az_h;  // fine slicing: slice centre height above each entry's atom
   std::vector<int> off_h((size_t)std::max(S, 1) * (nb + 1), 0);
   atoms_h.reserve(n);
   for (int s = 0; s < S; ++s) {
     const SliceRange& r = p->slices[s];
+    const double zmid = 0.5 * (r.z_start + r.z_end);
     std::vector<std::vector<long long>> bins(nb);
     for (long long i = r.i0; i <= r.i1; ++i) {
       int bx = (int)std::floor((p->ax[i] + bin) / bin), by = (int)std::floor((p->ay[i] + bin) / bin);
@@ -532,6 +569,7 @@ em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n
         int ti = t;
         std::memcpy(&a.w, &ti, sizeof(int));
         atoms_h.push_back(a);
+        if (p->fine) az_h.push_back((float)(zmid - p->az[i]));
       }
     }
     o[nb] = (int)atoms_h.size();
@@ -559,6 +597,47 @@ em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n
   EM_HIP(p, hipMemcpy(p->tables.p, tab_h.data(), sizeof(float) * tab_h.size(), hipMemcpyHostToDevice));
   EM_HIP(p, p->rmax2.ensure(sizeof(float) * rm_h.size()));
   EM_HIP(p, hipMemcpy(p->rmax2.p, rm_h.data(), sizeof(float) * rm_h.size(), hipMemcpyHostToDevice));
+  if (p->fine && !az_h.empty()) {
+    // per-entry splines of the potential integrated over each entry's slice
+    const long long ne = (long long)az_h.size();
+    std::vector<FineElement> el(nt);
+    std::vector<double> r2(std::max<size_t>(1, (size_t)nt * p->nR));
+    const double r_tap = p->tap * p->r_max;
+    for (int t = 0; t < nt; ++t) {
+      auto it = p->params3d.find(p->types[t]);
+      FineElement& e = el[t];
+      if (it != p->params3d.end()) {
+        e.pt = it->second.pt;
+        for (int k = 0; k < 6; ++k) {
+          e.cl[k] = it->second.cl[k];
+          e.cnl[k] = it->second.cnl[k];
+        }
+      } else {
+        e.pt = EM_PT_KIRKLAND_0_12;
+        moliere_3d(p->types[t], e.cl, e.cnl);
+      }
+      e.r2_tap = r_tap * r_tap;
+      e.tap_cf = 0.5 * kPi / (p->r_max * p->r_max - e.r2_tap);
+      for (int i = 0; i < p->nR; ++i) r2[(size_t)t * p->nR + i] = p->splines[t].R2[i];
+    }
+    std::vector<double> qx, qw;
+    gauss_legendre(p->nqz, qx, qw);
+    std::vector<double2> q(p->nqz);
+    for (int k = 0; k < p->nqz; ++k) q[k] = make_double2(qx[k], qw[k]);
+    EM_HIP(p, p->fine_az.ensure(sizeof(float) * ne));
+    EM_HIP(p, hipMemcpy(p->fine_az.p, az_h.data(), sizeof(float) * ne, hipMemcpyHostToDevice));
+    EM_HIP(p, p->fine_el.ensure(sizeof(FineElement) * el.size()));
+    EM_HIP(p, hipMemcpy(p->fine_el.p, el.data(), sizeof(FineElement) * el.size(), hipMemcpyHostToDevice));
+    EM_HIP(p, p->fine_r2.ensure(sizeof(double) * r2.size()));
+    EM_HIP(p, hipMemcpy(p->fine_r2.p, r2.data(), sizeof(double) * r2.size(), hipMemcpyHostToDevice));
+    EM_HIP(p, p->fine_q.ensure(sizeof(double2) * q.size()));
+    EM_HIP(p, hipMemcpy(p->fine_q.p, q.data(), sizeof(double2) * q.size(), hipMemcpyHostToDevice));
+    EM_HIP(p, p->fine_tab.ensure(sizeof(float4) * ne * p->nR));
+    EM_HIP(p, launch_fine_coefs(ne, p->nR, p->fine_r2.as<double>(), p->fine_el.as<FineElement>(),
+                                p->atoms.as<float4>(), p->fine_az.as<float>(), p->grid.dz, p->nqz,
+                                p->fine_q.as<double2>(), p->fine_tab.as<float4>(), p->stream));
+  }
+  p->fine_entries = p->fine ? (long long)az_h.size() : 0;
   p->has_specimen = true;
   p->vstore_slices = -1;
   p->graphs.clear();
@@ -604,7 +683,8 @@ em_status em_potential_slices(em_plan p, int s0, int ns, float* V) {
   const int nb = p->nbx * p->nby;
   EM_HIP(p, launch_potential(p->gdev, ns, p->off.as<int>() + (size_t)s0 * (nb + 1), p->nbx, p->nby,
                              p->bin, p->atoms.as<float4>(), (int)p->types.size(), p->nR,
-                             p->tables.as<float>(), p->rmax2.as<float>(), V, p->stream));
+                             p->tables.as<float>(), p->rmax2.as<float>(),
+                             p->fine_entries > 0 ? p->fine_tab.as<float4>() : nullptr, V, p->stream));
   return EM_OK;
 }
 
@@ -1129,6 +1209,28 @@ extern "C" em_status em_host_slicing(long long n, const double* z, double dz, in
     if (z[i] < z[i - 1]) return EM_ERR_INVALID;
   const std::vector<double> zz(z, z + n);
   const std::vector<SliceRange> s = make_slices(zz, dz, border, kZEps);
+  *n_slices = (int)s.size();
+  if ((int)s.size() > max_slices) return EM_ERR_INVALID;
+  for (size_t k = 0; k < s.size(); ++k) {
+    if (bounds) {
+      bounds[k] = s[k].z_start;
+      bounds[k + 1] = s[k].z_end;
+    }
+    if (i0) i0[k] = s[k].i0;
+    if (i1) i1[k] = s[k].i1;
+  }
+  return EM_OK;
+}
+
+extern "C" em_status em_host_slicing_fine(long long n, const double* z, double dz, int border, double r_int,
+                                          int max_slices, int* n_slices, double* bounds, long long* i0,
+                                          long long* i1) {
+  if (n < 0 || (n > 0 && !z) || !(dz > 0) || border < 0 || border > 2 || !(r_int >= 0) || !n_slices)
+    return EM_ERR_INVALID;
+  for (long long i = 1; i < n; ++i)
+    if (z[i] < z[i - 1]) return EM_ERR_INVALID;
+  const std::vector<double> zz(z, z + n);
+  const std::vector<SliceRange> s = make_slices_fine(zz, dz, border, kZEps, r_int);
   *n_slices = (int)s.size();
   if ((int)s.size() > max_slices) return EM_ERR_INVALID;
   for (size_t k = 0; k < s.size(); ++k) {

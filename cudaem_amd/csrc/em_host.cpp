@@ -157,6 +157,100 @@ std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int
   return out;
 }
 
+// Fine-sliced Slice_Generator (slice_generator.hpp:88-93, 108-128): bounds
+// over [z_min - r_int, z_max + r_int] (the integration range Atom_Coord_Data
+// reports, from a missing header: r_int = the largest spline radius), and
+// slice k takes the atoms with z in [min(z_mid - r_int, z_start) - eps,
+// max(z_mid + r_int, z_end) + eps] (Inc_Borders).
+std::vector<SliceRange> make_slices_fine(const std::vector<double>& z, double dz, int border, double eps,
+                                         double r_int) {
+  std::vector<SliceRange> out;
+  const long long n = (long long)z.size();
+  if (n == 0 || !(dz > 0)) return out;
+  const double vmin = z.front() - r_int, vmax = z.back() + r_int;
+  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
+  std::vector<double> b(nv);
+  if (border == EM_BORDER_MIN) {
+    for (int i = 0; i < nv; ++i) b[i] = (vmin - 2 * eps) + i * dz;
+  } else if (border == EM_BORDER_MAX) {
+    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = (vmax + 2 * eps) - i * dz;
+  } else {
+    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
+    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
+  }
+  out.resize(nv - 1);
+  for (int s = 0; s < nv - 1; ++s) {
+    SliceRange& r = out[s];
+    r.z_start = b[s];
+    r.z_end = b[s + 1];
+    const double zm = 0.5 * (b[s] + b[s + 1]);
+    const double zs = std::fmin(zm - r_int, b[s]) - eps, ze = std::fmax(zm + r_int, b[s + 1]) + eps;
+    if (zs > ze || z[n - 1] < zs || ze < z[0]) {
+      r.i0 = 1;
+      r.i1 = 0;
+      continue;
+    }
+    r.i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z.begin(), z.end(), zs) - z.begin());
+    r.i1 = (z[n - 1] <= ze) ? n - 1 : (long long)(std::lower_bound(z.begin(), z.end(), ze) - z.begin()) - 1;
+    if (r.i0 > r.i1 || z[r.i1] < zs || ze < z[r.i0]) {
+      r.i0 = 1;
+      r.i1 = 0;
+    }
+  }
+  return out;
+}
+
+// Newton iteration on the Legendre polynomial P_n from the usual cosine
+// guesses; nodes mapped to [-1/2, 1/2] with weights halved.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  auto legendre = [n](double t, double* pn, double* dpn) {
+    double p0 = 1.0, p1 = t;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    if (n == 1) {
+      p1 = t;
+      p0 = 1.0;
+    }
+    *pn = p1;
+    *dpn = n * (t * p1 - p0) / (t * t - 1.0);
+  };
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double t = std::cos(kPi * (i + 0.75) / (n + 0.5)), pn, dpn;
+    for (int it = 0; it < 100; ++it) {
+      legendre(t, &pn, &dpn);
+      const double dt = pn / dpn;
+      t -= dt;
+      if (std::fabs(dt) < 1e-16) break;
+    }
+    legendre(t, &pn, &dpn);
+    const double wi = 2.0 / ((1.0 - t * t) * dpn * dpn);
+    x[i] = -0.5 * t;
+    x[n - 1 - i] = 0.5 * t;
+    w[i] = 0.5 * wi;
+    w[n - 1 - i] = 0.5 * wi;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+void moliere_3d(int Z, double cl[6], double cnl[6]) {
+  static const double alpha[3] = {0.35, 0.55, 0.10};
+  static const double beta[3] = {0.3, 1.2, 6.0};
+  const double atf = 0.88534 * kA0 * std::pow((double)Z, -1.0 / 3.0);
+  for (int i = 0; i < 3; ++i) {
+    cl[i] = kEk * Z * alpha[i];
+    cnl[i] = beta[i] / atf;
+  }
+  for (int i = 3; i < 6; ++i) {
+    cl[i] = 0.0;
+    cnl[i] = 1.0;
+  }
+}
+
 double mrad_to_g2(double mrad, double lambda, bool outer) {
   if (outer && !(mrad > 0)) return HUGE_VAL;
   if (!outer && !(mrad > 0)) return 0.0;

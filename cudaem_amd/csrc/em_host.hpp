@@ -34,6 +34,16 @@ struct SliceRange {
 };
 // z sorted ascending.
 std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int border, double eps);
+// Fine-sliced mode: bounds over [z_min - r_int, z_max + r_int]; slice k holds
+// the atoms within r_int of its centre (and those inside it), widened by eps.
+std::vector<SliceRange> make_slices_fine(const std::vector<double>& z, double dz, int border, double eps,
+                                         double r_int);
+
+// Gauss-Legendre nodes and weights on [-1/2, 1/2], weights summing to 1.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
+// Moliere's screened potential as a 3D Kirkland 0-12 form (type 4):
+// cl_i = e^2/(4 pi eps0) Z alpha_i, cnl_i = beta_i / a_TF, no Gaussians.
+void moliere_3d(int Z, double cl[6], double cnl[6]);
 
 LensDev derive_lens(const em_lens& l, double lambda, double defocus_shift);
 // aperture / detector semi-angle (mrad) -> g^2 (1/A^2); outer <= 0 -> +inf

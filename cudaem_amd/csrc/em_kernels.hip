@@ -680,9 +680,11 @@ constexpr int kPotTile = 16;
 constexpr int kMaxTypes = 8;
 constexpr int kMaxNR = 256;
 
+template <bool FINE>
 __global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
     GridDev g, const int* __restrict__ off, int nbx, int nby, float bin, const float4* __restrict__ atoms,
-    int ntypes, int nR, const float* __restrict__ tables, const float* __restrict__ rmax2, float* V) {
+    int ntypes, int nR, const float* __restrict__ tables, const float* __restrict__ rmax2,
+    const float4* __restrict__ fine, float* V) {
   __shared__ float tab[kMaxTypes * 5 * kMaxNR];
   __shared__ float srm2[kMaxTypes];
   for (int i = threadIdx.x; i < ntypes * 5 * nR; i += blockDim.x) tab[i] = tables[i];
@@ -719,8 +721,15 @@ __global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
           if (r2 < R2[im]) ie = im; else i0 = im;
         }
         const float dx = r2 - R2[i0];
-        const float* cf = R2 + nR;
-        const float val = ((cf[3 * nR + i0] * dx + cf[2 * nR + i0]) * dx + cf[nR + i0]) * dx + cf[i0];
+        float val;
+        if constexpr (FINE) {
+          // the spline of this atom entry (its own slice height)
+          const float4 c = fine[(long long)ia * nR + i0];
+          val = ((c.w * dx + c.z) * dx + c.y) * dx + c.x;
+        } else {
+          const float* cf = R2 + nR;
+          val = ((cf[3 * nR + i0] * dx + cf[2 * nR + i0]) * dx + cf[nR + i0]) * dx + cf[i0];
+        }
         acc += at.z * val;
       }
     }
@@ -730,11 +739,137 @@ __global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
 
 hipError_t launch_potential(const GridDev& g, int nslices, const int* off, int nbx, int nby,
                             float bin, const float4* atoms, int ntypes, int nR,
-                            const float* tables, const float* rmax2, float* V, hipStream_t s) {
+                            const float* tables, const float* rmax2, const float4* fine, float* V,
+                            hipStream_t s) {
   if (ntypes > kMaxTypes || nR > kMaxNR || nslices <= 0) return hipErrorInvalidValue;
   const dim3 grid((g.nx + kPotTile - 1) / kPotTile, (g.ny + kPotTile - 1) / kPotTile, nslices);
-  hipLaunchKernelGGL(potential_kernel, grid, dim3(kPotTile * kPotTile), 0, s, g, off, nbx, nby, bin,
-                     atoms, ntypes, nR, tables, rmax2, V);
+  if (fine)
+    hipLaunchKernelGGL(potential_kernel<true>, grid, dim3(kPotTile * kPotTile), 0, s, g, off, nbx, nby, bin,
+                       atoms, ntypes, nR, tables, rmax2, fine, V);
+  else
+    hipLaunchKernelGGL(potential_kernel<false>, grid, dim3(kPotTile * kPotTile), 0, s, g, off, nbx, nby, bin,
+                       atoms, ntypes, nR, tables, rmax2, fine, V);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Fine slicing: the spline of each atom entry's potential integrated over its
+// slice (gen_sa_coeff_kernel + fill_spl_coeff_kernel, device_kernels.cuh:48-105).
+// One workgroup per entry, one thread per R^2 node; FP64 quadrature over the
+// Gauss-Legendre points z = dz x_q + a, weight dz w_q, then the taper of
+// apply_tapering_dev and the cubic coefficients of get_cubic_poly_coef.
+// --------------------------------------------------------------------------
+// V(r) and (1/r) dV/dr of the six parameterisations (Vr_dVrir_*,
+// basic_functions_dev.cuh:104-187), scaled by f.
+__device__ void vr3d(const FineElement& el, double r, double f, double& V, double& dVrir) {
+  double v = 0.0, d = 0.0;
+  const double r2 = r * r, ir = 1.0 / r;
+  switch (el.pt) {
+    case 1:
+    case 2:
+    case 3: {
+      const int n = el.pt == 1 ? 4 : 5;
+      for (int i = 0; i < n; ++i) {
+        const double e = el.cl[i] * exp(-el.cnl[i] * r2);
+        v += e;
+        d -= 2.0 * el.cnl[i] * e;
+      }
+      break;
+    }
+    case 4: {
+      for (int i = 0; i < 3; ++i) {
+        const double e = el.cl[i] * exp(-el.cnl[i] * r) * ir;
+        v += e;
+        d -= e * (el.cnl[i] + ir) * ir;
+      }
+      for (int i = 3; i < 6; ++i) {
+        const double e = el.cl[i] * exp(-el.cnl[i] * r2);
+        v += e;
+        d -= 2.0 * el.cnl[i] * e;
+      }
+      break;
+    }
+    case 5: {
+      double gsum = 0.0;
+      for (int i = 0; i < 6; ++i) {
+        v += el.cl[i] * erfc(el.cnl[i] * r);
+        gsum += el.cl[i] * el.cnl[i] * exp(-el.cnl[i] * el.cnl[i] * r2);
+      }
+      v *= ir;
+      d = -(1.1283791670955126 * gsum + v) / r2;  // 2 / sqrt(pi)
+      break;
+    }
+    case 6: {
+      double extra = 0.0;
+      for (int i = 0; i < 5; ++i) {
+        const double br = el.cnl[i] * r;
+        const double e = el.cl[i] * exp(-br);
+        v += e * (2.0 / br + 1.0);
+        extra += e * (br + 1.0);
+      }
+      d = -(v + extra) / r2;
+      break;
+    }
+    default:
+      break;
+  }
+  V = v * f;
+  dVrir = d * f;
+}
+
+__global__ __launch_bounds__(kMaxNR) void fine_coef_kernel(int nR, const double* __restrict__ R2,
+                                                          const FineElement* __restrict__ els,
+                                                          const float4* __restrict__ atoms,
+                                                          const float* __restrict__ az, double dz, int nq,
+                                                          const double2* __restrict__ q, float4* fine) {
+  __shared__ double s0[kMaxNR], s1[kMaxNR], sr[kMaxNR];
+  const long long e = blockIdx.x;
+  const int i = threadIdx.x;
+  const int t = __float_as_int(atoms[e].w);
+  const FineElement& el = els[t];
+  if (i < nR) {
+    const double x = R2[(long long)t * nR + i];
+    const double a = az[e];
+    double v = 0.0, dv = 0.0;
+    for (int k = 0; k < nq; ++k) {
+      const double z = dz * q[k].x + a;
+      double vq, dq;
+      vr3d(el, sqrt(z * z + x), dz * q[k].y, vq, dq);
+      v += vq;
+      dv += dq;
+    }
+    dv *= 0.5;
+    if (el.r2_tap < x) {
+      double sn, cs;
+      sincos(el.tap_cf * (x - el.r2_tap), &sn, &cs);
+      dv = dv * cs - el.tap_cf * v * sn;
+      v *= cs;
+    }
+    s0[i] = v;
+    s1[i] = dv;
+    sr[i] = x;
+  }
+  __syncthreads();
+  if (i < nR) {
+    double c2 = 0.0, c3 = 0.0;
+    if (i < nR - 1) {
+      const double idR = 1.0 / (sr[i + 1] - sr[i]);
+      const double m = (s0[i + 1] - s0[i]) * idR;
+      const double n = s1[i] + s1[i + 1];
+      c2 = (3.0 * m - n - s1[i]) * idR;
+      c3 = (n - 2.0 * m) * idR * idR;
+    }
+    fine[e * nR + i] = make_float4((float)s0[i], (float)s1[i], (float)c2, (float)c3);
+  }
+}
+
+hipError_t launch_fine_coefs(long long nentries, int nR, const double* R2, const FineElement* el,
+                             const float4* atoms, const float* az, double dz, int nq, const double2* q,
+                             float4* fine, hipStream_t s) {
+  if (nR > kMaxNR || nR < 2 || nq <= 0 || nq > kMaxFineQ || nentries <= 0 || nentries > 0x7fffffffLL)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fine_coef_kernel, dim3((unsigned)nentries), dim3((nR + 63) / 64 * 64), 0, s, nR, R2, el,
+                     atoms, az, dz, nq, q, fine);
   return hipGetLastError();
 }
 

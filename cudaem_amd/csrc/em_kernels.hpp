@@ -117,10 +117,28 @@ bool col_transposed_available(int ny);
 int col_ntiles(int nx, int ny);
 
 // --- other kernels ------------------------------------------------------------
+// fine: null, or one float4 (c0, c1, c2, c3) per spline node of every atom
+// entry (fine slicing: a spline per atom and slice instead of per element)
 hipError_t launch_potential(const GridDev& g, int nslices, const int* slice_bin_off,
                             int nbx, int nby, float bin_size, const float4* atoms,
                             int ntypes, int nR, const float* tables, const float* rmax2,
-                            float* V, hipStream_t s);
+                            const float4* fine, float* V, hipStream_t s);
+
+// 3D potential of one element for fine slicing: parameterisation type pt
+// (1 Doyle-Turner 0-4, 2 Peng 0-4, 3 Peng 0-12, 4 Kirkland 0-12,
+// 5 Weickenmeier-Kohl 0-12, 6 Lobato 0-12), coefficients, R^2 taper.
+struct FineElement {
+  int pt;
+  double cl[6], cnl[6];
+  double r2_tap, tap_cf;
+};
+constexpr int kMaxFineQ = 1024;
+// Spline of every atom entry e (type in atoms[e].w, slice centre height above
+// the atom az[e]) over its slice of thickness dz: nR nodes R2[type][i]
+// (double), nq Gauss-Legendre points (x, w) on [-1/2, 1/2] -> fine[e][nR].
+hipError_t launch_fine_coefs(long long nentries, int nR, const double* R2, const FineElement* el,
+                             const float4* atoms, const float* az, double dz, int nq,
+                             const double2* q, float4* fine, hipStream_t s);
 hipError_t launch_propagator(const GridDev& g, double w, double gx0, double gy0, int bwl,
                              float2* P, hipStream_t s);
 hipError_t launch_lens_table(const GridDev& g, const LensDev& l, float2* A, hipStream_t s);

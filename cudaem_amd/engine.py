@@ -158,6 +158,23 @@ class Plan:
     def set_spline_params(self, nR=128, r_min=0.005, r_max=3.0, tap=0.85):
         C.check(C.lib().em_set_spline_params(self._h, int(nR), r_min, r_max, tap), self._h)
 
+    def set_element_params(self, Z, ptype, cl, cnl):
+        """3D potential of element Z for fine slicing: parameterisation type
+        (C.PT_*) and up to 6 linear / non-linear coefficients."""
+        cl6 = np.zeros(6)
+        cnl6 = np.ones(6)
+        cl, cnl = np.asarray(cl, dtype=np.float64).ravel(), np.asarray(cnl, dtype=np.float64).ravel()
+        if len(cl) > 6 or len(cnl) != len(cl):
+            raise ValueError("up to 6 coefficient pairs")
+        cl6[:len(cl)] = cl
+        cnl6[:len(cnl)] = cnl
+        C.check(C.lib().em_set_element_params(self._h, int(Z), int(ptype), _dptr(cl6), _dptr(cnl6)), self._h)
+
+    def set_fine_slicing(self, enable=True, nq=128):
+        """Fine slicing from the next set_specimen: per (atom, slice) splines of
+        the 3D potential integrated over the slice with nq Gauss-Legendre points."""
+        C.check(C.lib().em_plan_set_fine_slicing(self._h, 1 if enable else 0, int(nq)), self._h)
+
     def set_specimen(self, x, y, z, Z, occ=None, border=C.BORDER_MIN):
         x, y, z = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z))
         Z = np.ascontiguousarray(Z, dtype=np.int32)

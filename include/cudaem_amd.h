@@ -140,6 +140,31 @@ em_status em_set_element_coefs(em_plan plan, int Z, int nk, const double* a, con
 /* Spline nodes: nR nodes log-spaced in R over [r_min, r_max], cosine taper
  * beyond tap_frac * r_max (defaults 128, 0.005, 3.0, 0.85). */
 em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max, double tap_frac);
+/* 3D potential parameterisations for fine slicing (Vr_dVrir_*,
+ * basic_functions_dev.cuh:104-214; the reference's ePotential_Type values are
+ * in a missing header, these numbers are the build's). */
+enum {
+  EM_PT_DOYLE_0_4 = 1,         /* 4 Gaussians cl exp(-cnl r^2) */
+  EM_PT_PENG_0_4 = 2,          /* 5 Gaussians */
+  EM_PT_PENG_0_12 = 3,         /* 5 Gaussians */
+  EM_PT_KIRKLAND_0_12 = 4,     /* 3 Yukawa cl exp(-cnl r)/r + 3 Gaussians */
+  EM_PT_WEICKENMEIER_0_12 = 5, /* 6 terms cl erfc(cnl r)/r */
+  EM_PT_LOBATO_0_12 = 6        /* 5 terms cl (2/(cnl r) + 1) exp(-cnl r) */
+};
+/* The 3D potential of element Z for fine slicing: type and 6 linear (cl, V A
+ * or V) and 6 non-linear (cnl, 1/A or 1/A^2) coefficients, unused terms
+ * ignored (the per-element Vr_coeff of device_containers.cuh:239, 280).
+ * Default for elements without one: Moliere's screening in the Kirkland form
+ * (3 Yukawa terms), whose projection is the default projected potential. */
+em_status em_set_element_params(em_plan plan, int Z, int ptype, const double* cl, const double* cnl);
+/* Fine slicing on / off (multislice_params.enable_fine_slicing,
+ * projected_potential.cuh:33-44, 80-88) with nq Gauss-Legendre points per
+ * slice (default 128, at most 1024).  Applies from the next
+ * em_plan_set_specimen: slices then span [z_min - R_max, z_max + R_max], each
+ * holding the atoms within R_max of its centre, and every (atom, slice) pair
+ * gets the spline of the atom's 3D potential integrated over that slice
+ * (gen_sa_coeff_kernel + fill_spl_coeff_kernel, device_kernels.cuh:48-105). */
+em_status em_plan_set_fine_slicing(em_plan plan, int enable, int nq);
 /* Copies the atoms, sorts them by z, slices them (Specimen::set_input_data,
  * specimen.cuh:16-36 + Slice_Generator::calculate, slice_generator.hpp:48-168)
  * and uploads the binned atom list.  Returns the slice count. */
@@ -162,6 +187,12 @@ em_status em_spline_tables(em_plan plan, int max_types, int* Zs, int* ntypes, in
  * Returns EM_ERR_INVALID when more than max_slices slices would be made. */
 em_status em_host_slicing(long long n, const double* z_sorted, double dz, int border, int max_slices,
                           int* n_slices, double* bounds, long long* iatom_start, long long* iatom_end);
+/* Fine-sliced slicing (slice_generator.hpp:88-93, 108-128), host only: as
+ * em_host_slicing over [z_min - r_int, z_max + r_int] with atom windows
+ * [min(z_mid - r_int, z_start) - eps, max(z_mid + r_int, z_end) + eps]. */
+em_status em_host_slicing_fine(long long n, const double* z_sorted, double dz, int border, double r_int,
+                               int max_slices, int* n_slices, double* bounds, long long* iatom_start,
+                               long long* iatom_end);
 /* Spline of the projected potential of element Z (default coefficients) in
  * R^2: R2, c0..c3 each [nR] (basic_functions_dev.cuh:68-101 conventions). */
 em_status em_host_spline(int Z, int nR, double r_min, double r_max, double tap_frac, double* R2,

@@ -26,6 +26,7 @@ class OracleSpecimen:
         a = sort_atoms(atoms)
         self.atoms = a
         self.nx, self.ny, self.lx, self.ly, self.dz = nx, ny, lx, ly, dz
+        self._border = border
         self.bounds, self.i0, self.i1 = O.slicing(a["z"], dz, border)
         self.S = len(self.i0)
         self.types = sorted(set(int(z) for z in a["Z"]))
@@ -39,6 +40,30 @@ class OracleSpecimen:
 
     def potentials(self):
         return np.stack([self.potential(s) for s in range(self.S)])
+
+    def use_fine_slicing(self, params=None, border=None, r_int=None):
+        """Switch to fine slicing: params maps Z -> (type, cl, cnl); the
+        default is Moliere's 3D form."""
+        a = self.atoms
+        r_int = O.SPLINE_RMAX if r_int is None else r_int
+        border = self._border if border is None else border
+        self.bounds, self.i0, self.i1 = O.slicing_fine(a["z"], self.dz, border, r_int)
+        self.S = len(self.i0)
+        params = params or {}
+        self.fine = []
+        for Z in self.types:
+            if Z in params:
+                self.fine.append(params[Z])
+            else:
+                cl, cnl = O.moliere_3d(Z)
+                self.fine.append((O.PT_KIRKLAND_0_12, cl, cnl))
+
+    def potential_fine(self, s):
+        a = self.atoms
+        pts = [f[0] for f in self.fine]
+        return O.potential_fine(self.nx, self.ny, self.lx, self.ly, self.bounds[s], self.bounds[s + 1],
+                                self.i0[s], self.i1[s], a["x"], a["y"], a["z"], a["occ"], self.tidx, pts,
+                                [f[1] for f in self.fine], [f[2] for f in self.fine], self.vr)
 
     def zmid0(self):
         return 0.5 * (self.bounds[0] + self.bounds[1])

@@ -104,6 +104,29 @@ def test_host_slicing_matches_oracle(C, border):
             assert np.array_equal(b, bo) and np.array_equal(i0, i0o) and np.array_equal(i1, i1o)
 
 
+@pytest.mark.parametrize("border", [0, 1, 2])
+def test_host_slicing_fine_matches_oracle(C, border):
+    """Fine-sliced bounds and atom windows equal the oracle's bit for bit."""
+    import cudaem_amd.specimen as sp
+    rng = np.random.default_rng(10 + border)
+    cases = [np.sort(rng.uniform(-5, 30, 700)), np.sort(sp.vibrate(sp.srtio3(2, 2, 10), 0.08, 5, 1)["z"]),
+             np.array([1.0])]
+    cap = 4096
+    for z in cases:
+        for dz, r_int in ((1.9525, 3.0), (1.35775, 2.5), (0.7, 0.0)):
+            ns = ctypes.c_int()
+            b = np.zeros(cap + 1)
+            i0 = np.zeros(cap, np.int64)
+            i1 = np.zeros(cap, np.int64)
+            st = C.lib().em_host_slicing_fine(len(z), z.ctypes.data_as(C._PD), dz, border, r_int, cap,
+                                              ctypes.byref(ns), b.ctypes.data_as(C._PD),
+                                              i0.ctypes.data_as(C._PLL), i1.ctypes.data_as(C._PLL))
+            assert st == 0
+            S = ns.value
+            bo, i0o, i1o = O.slicing_fine(z, dz, border, r_int)
+            assert np.array_equal(b[:S + 1], bo) and np.array_equal(i0[:S], i0o) and np.array_equal(i1[:S], i1o)
+
+
 def test_host_slicing_rejects_unsorted(C):
     z = np.array([1.0, 0.5])
     ns = ctypes.c_int()

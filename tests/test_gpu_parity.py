@@ -86,6 +86,59 @@ def test_potential_matches_oracle(em):
     assert rel_l2(V, Vo) < TOL_OP
 
 
+def test_fine_sliced_potential_matches_oracle(em):
+    """Fine slicing: slices span z_min - R_max .. z_max + R_max; every atom
+    entry carries the spline of its 3D potential integrated over its slice,
+    built on the device in FP64 (default Moliere for O and Ti, Lobato for Sr)."""
+    import cudaem_amd.specimen as sp
+    import torch
+    at = sp.vibrate(sp.srtio3(2, 2, 2), 0.08, 11, 1)
+    n = 128
+    lob = (O.PT_LOBATO_0_12, [1.0, 3.0, 8.0, 15.0, 4.0], [4.0, 6.0, 10.0, 20.0, 40.0])
+    with em.Plan(n, n, at["lx"], at["ly"], 1.9525, 300.0) as p:
+        p.set_fine_slicing(True)
+        p.set_element_params(38, lob[0], lob[1], lob[2])
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        V = _np(p.potential_slices())
+        zs, ze, i0, i1 = p.slice_info(1)
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], 1.9525, 300.0, O.BORDER_MIN)
+    o.use_fine_slicing({38: lob})
+    assert S == o.S
+    assert (zs, ze, i0, i1) == (o.bounds[1], o.bounds[2], o.i0[1], o.i1[1])
+    Vo = np.stack([o.potential_fine(s) for s in range(S)])
+    assert rel_l2(V, Vo) < TOL_OP
+    # the potential reaches R_max beyond the outermost atoms: more slices
+    # than direct slicing makes, every one of them non-empty
+    assert S >= len(O.slicing(np.sort(at["z"]), 1.9525, O.BORDER_MIN)[1]) + 2
+    assert (np.abs(V).reshape(S, -1).max(axis=1) > 0).all()
+
+
+def test_fine_sliced_hrtem_matches_oracle(em):
+    """Plane-wave exit wave through a fine-sliced specimen (HRTEM border)."""
+    import cudaem_amd.specimen as sp
+    at = sp.fcc_cu(3, 3, 3)
+    n = 128
+    dz, e0 = 1.8075, 200.0
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=1) as p:
+        p.set_fine_slicing(True, nq=64)
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MAX)
+        V = p.potential_slices()
+        psi = p.plane_wave(1)
+        p.multislice_forward(psi, V=V)
+        Psi = _np(psi)[0]
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MAX)
+    o.use_fine_slicing()
+    assert S == o.S
+    Vo = np.stack([O.potential_fine(n, n, at["lx"], at["ly"], o.bounds[s], o.bounds[s + 1], o.i0[s], o.i1[s],
+                                    o.atoms["x"], o.atoms["y"], o.atoms["z"], o.atoms["occ"], o.tidx,
+                                    [f[0] for f in o.fine], [f[1] for f in o.fine], [f[2] for f in o.fine],
+                                    o.vr, nq=64) for s in range(S)])
+    Po = o.propagator()
+    Psio = O.multislice(Po, np.ones((n, n), complex), V=Vo)
+    assert rel_l2(Psi, Psio) < TOL_WAVE
+
+
 def test_spline_tables_match_oracle(em):
     import cudaem_amd.specimen as sp
     at = sp.srtio3(2, 2, 2)
