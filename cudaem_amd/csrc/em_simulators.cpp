@@ -47,6 +47,20 @@ struct Plan {
   }
 };
 
+// Potential options of a run: fine slicing and the elements' 3D coefficients.
+template <class T>
+void configure_potential(em_plan p, const multislice_params<T>& m) {
+  for (const Element_Potential<T>& e : m.element_potentials) {
+    double cl[6], cnl[6];
+    for (int i = 0; i < 6; ++i) {
+      cl[i] = e.cl[i];
+      cnl[i] = e.cnl[i];
+    }
+    ok(em_set_element_params(p, e.Z, (int)m.potential_type, cl, cnl), p, "em_set_element_params");
+  }
+  ok(em_plan_set_fine_slicing(p, m.enable_fine_slicing ? 1 : 0, m.fine_slicing_nq), p, "em_plan_set_fine_slicing");
+}
+
 template <class T>
 em_grid grid_of(const multislice_params<T>& m) {
   em_grid g;
@@ -226,6 +240,7 @@ void run_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T>* o)
   const multislice_params<T>& m = *mp;
   m.validate_parameters();
   Plan plan(grid_of(m), beam_of(m), 1);
+  configure_potential(plan.p, m);
   const size_t N = (size_t)m.nx * m.ny;
   const bool full = (m.data_record == eWave);
   const int border = (m.simulation_type == eTEMST_HRTEM) ? EM_BORDER_MAX : EM_BORDER_MIN;
@@ -298,6 +313,7 @@ void run_STEM_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T
   phonon_range(m, &nconf, &w);
   const bool tile = (m.stem_scanning == eTile_scanning);
   Plan plan(grid_of(m), beam_of(m), tile ? std::max(1, m.probe_batch) : 1);
+  configure_potential(plan.p, m);
   o->init(m.STEM_row, m.STEM_col, nd, false);
   o->interval_x = (m.scan_range[2] - m.scan_range[0]) * m.lx / m.STEM_col;
   o->interval_y = (m.scan_range[3] - m.scan_range[1]) * m.ly / m.STEM_row;
@@ -359,6 +375,7 @@ void run_thickness_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_D
   m.validate_parameters();
   if ((int)m.detector.size() <= m.thickness_det) throw simulation_error("thickness filter detector missing");
   Plan plan(grid_of(m), beam_of(m), 1);
+  configure_potential(plan.p, m);
   const size_t N = (size_t)m.nx * m.ny;
   int nconf;
   float w;

@@ -31,6 +31,16 @@ enum eIllumination_Model { eIM_Coherent = 1, eIM_Partial_Coherent = 2, eIM_Trans
 enum eScanning_Type { eTile_scanning = 1, eGeneral_scanning = 2 };
 enum eData_Record { eIntensity = 1, eWave = 2 };
 enum eMatch_Border { eMB_Min = 0, eMB_Max = 1, eMB_Middle = 2 };
+// 3D potential parameterisations of fine slicing (Vr_dVrir_dev,
+// basic_functions_dev.cuh:189-214); values as EM_PT_* of cudaem_amd.h.
+enum ePotential_Type {
+  ePT_Doyle_0_4 = 1,
+  ePT_Peng_0_4 = 2,
+  ePT_Peng_0_12 = 3,
+  ePT_Kirkland_0_12 = 4,
+  ePT_Weickenmeier_0_12 = 5,
+  ePT_Lobato_0_12 = 6
+};
 
 struct simulation_error : std::runtime_error {
   explicit simulation_error(const std::string& m) : std::runtime_error(m) {}
@@ -71,6 +81,15 @@ struct Detector {
   std::size_t size() const { return ang_inner.size(); }
 };
 
+// 3D potential coefficients of one element (the reference reads them from
+// its missing element tables into Vr_coeff, device_containers.cuh:239, 280).
+template <class T>
+struct Element_Potential {
+  int Z = 0;
+  T cl[6] = {0, 0, 0, 0, 0, 0};
+  T cnl[6] = {1, 1, 1, 1, 1, 1};
+};
+
 template <class T>
 struct multislice_params {
   eTEM_Sim_Type simulation_type = eTEMST_HRTEM;
@@ -88,6 +107,14 @@ struct multislice_params {
   eIllumination_Model illumination_model = eIM_Coherent;
   // specimen
   Atom_Coord_Data<T> atoms;
+  // fine slicing (multislice_params_dev.cuh:23-24): each element's 3D
+  // potential of type potential_type integrated over every slice it reaches;
+  // elements without an entry in element_potentials use Moliere's screening
+  // in the Kirkland form
+  bool enable_fine_slicing = false;
+  ePotential_Type potential_type = ePT_Kirkland_0_12;
+  std::vector<Element_Potential<T>> element_potentials;
+  int fine_slicing_nq = 128;  // Gauss-Legendre points per slice
   // phonons: Gaussian displacements of rms pn_sigma along enabled pn_dim
   ePhonon_Model phonon_model = ePM_Still_Atom;
   int pn_nconf = 1;

@@ -91,6 +91,27 @@ int main(int argc, char** argv) {
       write(out + "/ed_im.f32", o.image_imag);
       std::printf("ed %zu\n", o.image_real.size());
     }
+    {  // ED, full wave, fine slicing; Sr in the Lobato form, O and Ti default
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_ED;
+      p.data_record = eWave;
+      p.enable_fine_slicing = true;
+      p.fine_slicing_nq = 64;
+      p.potential_type = ePT_Lobato_0_12;
+      Element_Potential<float> sr;
+      sr.Z = 38;
+      const float cl[5] = {1.f, 3.f, 8.f, 15.f, 4.f}, cnl[5] = {4.f, 6.f, 10.f, 20.f, 40.f};
+      for (int i = 0; i < 5; ++i) {
+        sr.cl[i] = cl[i];
+        sr.cnl[i] = cnl[i];
+      }
+      p.element_potentials.push_back(sr);
+      output_Multi_Data<float> o;
+      sim.executionTEM(p, &o);
+      write(out + "/ed_fine_re.f32", o.image_real);
+      write(out + "/ed_fine_im.f32", o.image_imag);
+      std::printf("ed_fine %zu\n", o.image_real.size());
+    }
     {  // CBED at (5, 6), two frozen-phonon configurations
       multislice_params<float> p = base(128, 3, 4);
       p.simulation_type = eTEMST_CBED;

@@ -72,6 +72,21 @@ def test_cpp_ed_full_wave(outputs):
     assert rel_l2(got, shift(Psi.real) + 1j * shift(Psi.imag)) < 1e-4
 
 
+def test_cpp_ed_fine_slicing(outputs):
+    """ED exit wave through the fine-sliced specimen, Sr given in the Lobato
+    form through multislice_params.element_potentials."""
+    at = atoms_f32()
+    o = OracleSpecimen(at, N, N, L, L, DZ, E0, O.BORDER_MIN)
+    o.use_fine_slicing({38: (O.PT_LOBATO_0_12, [1.0, 3.0, 8.0, 15.0, 4.0], [4.0, 6.0, 10.0, 20.0, 40.0])})
+    a = o.atoms
+    Vo = np.stack([O.potential_fine(N, N, L, L, o.bounds[s], o.bounds[s + 1], o.i0[s], o.i1[s], a["x"], a["y"],
+                                    a["z"], a["occ"], o.tidx, [f[0] for f in o.fine], [f[1] for f in o.fine],
+                                    [f[2] for f in o.fine], o.vr, nq=64) for s in range(o.S)])
+    Psi = O.multislice(o.propagator(), np.ones((N, N), complex), V=Vo)
+    got = outputs("ed_fine_re.f32").reshape(N, N) + 1j * outputs("ed_fine_im.f32").reshape(N, N)
+    assert rel_l2(got, shift(Psi.real) + 1j * shift(Psi.imag)) < 1e-4
+
+
 def test_cpp_cbed_frozen_phonons(outputs):
     at = atoms_f32()
     acc = np.zeros((N, N))
