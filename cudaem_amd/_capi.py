@@ -34,8 +34,8 @@ EXPORTS = [
     "em_plan_set_specimen", "em_slice_info",
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_adjoint",
-    "em_detector_sums", "em_ctf_image", "em_diffraction", "em_stem", "em_thickness",
-    "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_spline",
+    "em_detector_sums", "em_ctf_image", "em_pctf_image", "em_tcc_image", "em_diffraction", "em_stem", "em_thickness",
+    "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_quadrature", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
     "em_plan_set_band_skip",
 ]
@@ -52,6 +52,15 @@ class em_beam(ctypes.Structure):
 
 class em_lens(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in LENS_FIELDS]
+
+
+# partial coherence (em_coherence of include/cudaem_amd.h)
+COH_TEMPORAL_SPATIAL, COH_TEMPORAL, COH_SPATIAL = 1, 2, 3
+
+
+class em_coherence(ctypes.Structure):
+    _fields_ = [("dsf_sigma", ctypes.c_double), ("dsf_npoints", ctypes.c_int), ("ssf_beta", ctypes.c_double),
+                ("ssf_npoints", ctypes.c_int), ("mode", ctypes.c_int)]
 
 
 class em_tuning(ctypes.Structure):
@@ -104,6 +113,8 @@ _SIGS = {
     "em_multislice_adjoint": ([_P, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _I),
     "em_detector_sums": ([_P, _P, _I, _I, _PD, _PD, _F, _I, _P], _I),
     "em_ctf_image": ([_P, ctypes.POINTER(em_lens), _P, _I, _P, _P, _P, _F, _I], _I),
+    "em_pctf_image": ([_P, ctypes.POINTER(em_lens), ctypes.POINTER(em_coherence), _P, _I, _P, _P, _P, _F, _I], _I),
+    "em_tcc_image": ([_P, ctypes.POINTER(em_lens), ctypes.POINTER(em_coherence), _P, _I, _P, _P, _P, _F, _I], _I),
     "em_diffraction": ([_P, _P, _I, _P, _F, _I], _I),
     "em_stem": ([_P, ctypes.POINTER(em_lens), _I, _PD, _PD, _I, _PD, _PD, _F, _I, _P], _I),
     "em_thickness": ([_P, _P, _I, _P, _I, _D, _D, _I, _F, _I, _P, _PI], _I),
@@ -112,6 +123,7 @@ _SIGS = {
     "em_time_passes": ([_P, _P, _P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),
     "em_host_slicing": ([ctypes.c_longlong, _PD, _D, _I, _I, _PI, _PD, _PLL, _PLL], _I),
     "em_host_slicing_fine": ([ctypes.c_longlong, _PD, _D, _I, _D, _I, _PI, _PD, _PLL, _PLL], _I),
+    "em_host_quadrature": ([_I, _I, _PD, _PD], _I),
     "em_host_spline": ([_I, _I, _D, _D, _D, _PD, _PD, _PD, _PD, _PD], _I),
     "em_host_constants": ([_D, _PD, _PD, _PD], _I),
     "em_plan_get_tuning": ([_P, ctypes.POINTER(em_tuning)], _I),

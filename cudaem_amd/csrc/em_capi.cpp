@@ -857,13 +857,16 @@ em_status em_detector_sums(em_plan p, const void* Psi, int nprobe, int ndet, con
   return EM_OK;
 }
 
-em_status em_ctf_image(em_plan p, const em_lens* obj, const void* Psi, int nprobe, float* I, float* re,
-                       float* im, float w, int accumulate) {
-  if (!p || !obj || !Psi || nprobe <= 0 || (!I && !(re && im))) return EM_ERR_INVALID;
-  const long long N = p->N();
-  EM_HIP(p, p->A.ensure(sizeof(float2) * N));
-  EM_HIP(p, launch_lens_table(p->gdev, derive_lens(*obj, p->k.lambda, 0.0), p->A.as<float2>(), p->stream));
-  EM_HIP(p, p->tmp.ensure(sizeof(float2) * N * nprobe));
+}  // extern "C"
+
+namespace {
+
+// Image of Psi through the lens table in p->A (apply_CTF_kernel +
+// inverse FFT + wave_amplitude / wave_spliter): column pass Psi * A then
+// IFFT_y, row pass IFFT_x then w |.|^2 or w (re, im), stored or accumulated.
+em_status image_from_table(em_plan p, const void* Psi, int nprobe, float* I, float* re, float* im, float w,
+                           int accumulate) {
+  EM_HIP(p, p->tmp.ensure(sizeof(float2) * p->N() * nprobe));
   ColArgs c = col_args(p, reinterpret_cast<const float2*>(Psi), p->tmp.as<float2>(), nprobe);
   c.A = p->A.as<float2>();
   EM_TRY(col(p, COL_LOAD_MUL_A | COL_POST_IFFT | COL_STORE, c));
@@ -878,7 +881,81 @@ em_status em_ctf_image(em_plan p, const em_lens* obj, const void* Psi, int nprob
     r.o2 = im;
     m |= ROW_OUT_SPLIT;
   }
-  EM_TRY(row(p, m, r));
+  return row(p, m, r);
+}
+
+bool coherence_valid(const em_coherence* c, bool tcc) {
+  if (!c || !(c->dsf_sigma >= 0) || !(c->ssf_beta >= 0) || c->mode < EM_COH_TEMPORAL_SPATIAL ||
+      c->mode > EM_COH_SPATIAL)
+    return false;
+  if (tcc) {
+    const bool t = c->mode != EM_COH_SPATIAL, s = c->mode != EM_COH_TEMPORAL;
+    if (t && (c->dsf_npoints < 1 || c->dsf_npoints > 64)) return false;
+    if (s && (c->ssf_npoints < 1 || c->ssf_npoints > 64)) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+em_status em_ctf_image(em_plan p, const em_lens* obj, const void* Psi, int nprobe, float* I, float* re,
+                       float* im, float w, int accumulate) {
+  if (!p || !obj || !Psi || nprobe <= 0 || (!I && !(re && im))) return EM_ERR_INVALID;
+  EM_HIP(p, p->A.ensure(sizeof(float2) * p->N()));
+  EM_HIP(p, launch_lens_table(p->gdev, derive_lens(*obj, p->k.lambda, 0.0), p->A.as<float2>(), p->stream));
+  return image_from_table(p, Psi, nprobe, I, re, im, w, accumulate);
+}
+
+em_status em_pctf_image(em_plan p, const em_lens* obj, const em_coherence* coh, const void* Psi, int nprobe,
+                        float* I, float* re, float* im, float w, int accumulate) {
+  if (!p || !obj || !Psi || nprobe <= 0 || (!I && !(re && im)) || !coherence_valid(coh, false))
+    return EM_ERR_INVALID;
+  LensDev l = derive_lens(*obj, p->k.lambda, 0.0);
+  // PCTF_TEM drops the spread that the mode leaves out (lens_modulation.cuh:95-102)
+  l.envelope = 1;
+  l.env_lambda = p->k.lambda;
+  l.env_c10 = obj->c_10;
+  l.env_c30 = obj->c_30;
+  l.env_sigma = coh->mode == EM_COH_SPATIAL ? 0.0 : coh->dsf_sigma;
+  l.env_beta = coh->mode == EM_COH_TEMPORAL ? 0.0 : coh->ssf_beta * 1e-3;
+  EM_HIP(p, p->A.ensure(sizeof(float2) * p->N()));
+  EM_HIP(p, launch_lens_table(p->gdev, l, p->A.as<float2>(), p->stream));
+  return image_from_table(p, Psi, nprobe, I, re, im, w, accumulate);
+}
+
+em_status em_tcc_image(em_plan p, const em_lens* obj, const em_coherence* coh, const void* Psi, int nprobe,
+                       float* I, float* re, float* im, float w, int accumulate) {
+  if (!p || !obj || !Psi || nprobe <= 0 || (!I && !(re && im)) || !coherence_valid(coh, true))
+    return EM_ERR_INVALID;
+  const bool temporal = coh->mode != EM_COH_SPATIAL, spatial = coh->mode != EM_COH_TEMPORAL;
+  std::vector<double> tx{0.0}, tw{1.0}, sx{0.0}, sw{1.0};
+  if (temporal) {
+    gauss_hermite(coh->dsf_npoints, tx, tw);
+    for (double& v : tw) v /= std::sqrt(kPi);
+  }
+  if (spatial) {
+    gauss_hermite(coh->ssf_npoints, sx, sw);
+    for (double& v : sw) v /= std::sqrt(kPi);
+  }
+  const double q0 = std::sin(coh->ssf_beta * 1e-3) / p->k.lambda;
+  EM_HIP(p, p->A.ensure(sizeof(float2) * p->N()));
+  bool first = true;
+  for (size_t a = 0; a < sx.size(); ++a)
+    for (size_t b = 0; b < sx.size(); ++b)
+      for (size_t j = 0; j < tx.size(); ++j) {
+        // TCC_TEM's node loops: defocus set per temporal node, tilt per source node
+        em_lens lj = *obj;
+        lj.c_10 = obj->c_10 + std::sqrt(2.0) * coh->dsf_sigma * tx[j];
+        LensDev l = derive_lens(lj, p->k.lambda, 0.0);
+        l.gxs = q0 * sx[a];
+        l.gys = q0 * sx[b];
+        EM_HIP(p, launch_lens_table(p->gdev, l, p->A.as<float2>(), p->stream));
+        const float wn = (float)(w * sw[a] * sw[b] * tw[j]);
+        EM_TRY(image_from_table(p, Psi, nprobe, I, re, im, wn, first ? accumulate : 1));
+        first = false;
+      }
   return EM_OK;
 }
 
@@ -1218,6 +1295,17 @@ extern "C" em_status em_host_slicing(long long n, const double* z, double dz, in
     }
     if (i0) i0[k] = s[k].i0;
     if (i1) i1[k] = s[k].i1;
+  }
+  return EM_OK;
+}
+
+extern "C" em_status em_host_quadrature(int kind, int n, double* x, double* w) {
+  if (n < 1 || n > 4096 || !x || !w || kind < 0 || kind > 1) return EM_ERR_INVALID;
+  std::vector<double> xs, ws;
+  if (kind == 0) gauss_legendre(n, xs, ws); else gauss_hermite(n, xs, ws);
+  for (int i = 0; i < n; ++i) {
+    x[i] = xs[i];
+    w[i] = ws[i];
   }
   return EM_OK;
 }

@@ -237,6 +237,47 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
   if (n % 2 == 1) x[n / 2] = 0.0;
 }
 
+// Newton iteration on the orthonormal Hermite recurrence
+// p_{j+1} = t sqrt(2/(j+1)) p_j - sqrt(j/(j+1)) p_{j-1}, p_0 = pi^(-1/4),
+// from the usual asymptotic starting points (largest node first); weight
+// 2 / p_n'(t)^2 with p_n' = sqrt(2 n) p_{n-1}.
+void gauss_hermite(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const int m = (n + 1) / 2;
+  std::vector<double> r(m), rw(m);
+  double t = 0.0;
+  for (int i = 0; i < m; ++i) {
+    if (i == 0) t = std::sqrt(2.0 * n + 1.0) - 1.85575 * std::pow(2.0 * n + 1.0, -0.16667);
+    else if (i == 1) t -= 1.14 * std::pow((double)n, 0.426) / t;
+    else if (i == 2) t = 1.86 * t - 0.86 * r[0];
+    else if (i == 3) t = 1.91 * t - 0.91 * r[1];
+    else t = 2.0 * t - r[i - 2];
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 0.7511255444649425, p0 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double p2 = t * std::sqrt(2.0 / (j + 1)) * p1 - std::sqrt((double)j / (j + 1)) * p0;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = std::sqrt(2.0 * n) * p0;
+      const double dt = p1 / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-15) break;
+    }
+    r[i] = t;
+    rw[i] = 2.0 / (dp * dp);
+  }
+  for (int i = 0; i < m; ++i) {
+    x[i] = -r[i];
+    w[i] = rw[i];
+    x[n - 1 - i] = r[i];
+    w[n - 1 - i] = rw[i];
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
 void moliere_3d(int Z, double cl[6], double cnl[6]) {
   static const double alpha[3] = {0.35, 0.55, 0.10};
   static const double beta[3] = {0.3, 1.2, 6.0};

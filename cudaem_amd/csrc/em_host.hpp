@@ -41,6 +41,8 @@ std::vector<SliceRange> make_slices_fine(const std::vector<double>& z, double dz
 
 // Gauss-Legendre nodes and weights on [-1/2, 1/2], weights summing to 1.
 void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
+// Gauss-Hermite nodes and weights for the weight exp(-t^2), ascending.
+void gauss_hermite(int n, std::vector<double>& x, std::vector<double>& w);
 // Moliere's screened potential as a 3D Kirkland 0-12 form (type 4):
 // cl_i = e^2/(4 pi eps0) Z alpha_i, cnl_i = beta_i / a_TF, no Gaussians.
 void moliere_3d(int Z, double cl[6], double cnl[6]);

@@ -940,13 +940,23 @@ __global__ void lens_table_kernel(GridDev g, LensDev l, float2* A) {
     const int ix = (int)(i % g.nx), iy = (int)(i / g.nx);
     const int sx = ix < g.nx / 2 ? ix : ix - g.nx;
     const int sy = iy < g.ny / 2 ? iy : iy - g.ny;
-    const double gx = __dmul_rn((double)sx, g.dgx), gy = __dmul_rn((double)sy, g.dgy);
+    const double gx = __dadd_rn(__dmul_rn((double)sx, g.dgx), l.gxs);
+    const double gy = __dadd_rn(__dmul_rn((double)sy, g.dgy), l.gys);
     const double g2 = __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
     float2 v = make_float2(0.f, 0.f);
     if (l.g2_min <= g2 && g2 < l.g2_max) {
       double sn, cs;
       sincos(lens_chi(l, gx, gy), &sn, &cs);
-      v = make_float2((float)cs, (float)sn);
+      double amp = 1.0;
+      if (l.envelope) {
+        // u = 1 + 2 (pi beta sigma)^2 g^2; E = exp(-(spatial + temporal) / u) / sqrt(u)
+        const double c = kPiD * l.env_beta * l.env_sigma;
+        const double u = 1.0 + 2.0 * c * c * g2;
+        const double tc = kPiD * l.env_lambda * l.env_sigma * g2;
+        const double sc = kPiD * l.env_beta * (l.env_c30 * l.env_lambda * l.env_lambda * g2 + l.env_c10);
+        amp = exp(-(sc * sc * g2 + 0.5 * tc * tc) / u) / sqrt(u);
+      }
+      v = make_float2((float)(amp * cs), (float)(amp * sn));
     }
     A[i] = v;
   }

@@ -21,6 +21,13 @@ struct LensDev {
   double phi12, phi21, phi23, phi32, phi34, phi41, phi43, phi45, phi52, phi54, phi56;
   double g2_min, g2_max;
   int phi_required;
+  // tilt added to g (apply_CTF_kernel's gxu, gyu, device_kernels.cuh:446-454)
+  double gxs = 0, gys = 0;
+  // quasi-coherent envelope of the PCTF (incoherent_phase_shift,
+  // basic_functions_dev.cuh:236-266): on/off, wavelength, defocus and C30 (A),
+  // defocus-spread standard deviation (A), source half-width angle (rad)
+  int envelope = 0;
+  double env_lambda = 0, env_c10 = 0, env_c30 = 0, env_sigma = 0, env_beta = 0;
 };
 
 // --- row pass ---------------------------------------------------------------

@@ -339,6 +339,38 @@ class Plan:
                    1 if accumulate else 0)
         return I
 
+    def _coherent_image(self, fn, lens, coh, Psi, full_wave, w, out, accumulate):
+        torch = _torch()
+        B = Psi.shape[0]
+        _need(Psi, (B, self.ny, self.nx), torch.complex64, "Psi")
+        ls = lens_struct(lens)
+        cs = C.em_coherence(float(coh.get("dsf_sigma", 0.0)), int(coh.get("dsf_npoints", 1)),
+                            float(coh.get("ssf_beta", 0.0)), int(coh.get("ssf_npoints", 1)),
+                            int(coh.get("mode", C.COH_TEMPORAL_SPATIAL)))
+        if full_wave:
+            re, im = out if out is not None else (
+                torch.zeros((B, self.ny, self.nx), dtype=torch.float32, device="cuda"),
+                torch.zeros((B, self.ny, self.nx), dtype=torch.float32, device="cuda"))
+            _need(re, (B, self.ny, self.nx), torch.float32, "re")
+            _need(im, (B, self.ny, self.nx), torch.float32, "im")
+            self._call(fn, ctypes.byref(ls), ctypes.byref(cs), _ptr(Psi), B, None, _ptr(re), _ptr(im), float(w),
+                       1 if accumulate else 0)
+            return re, im
+        I = out if out is not None else torch.zeros((B, self.ny, self.nx), dtype=torch.float32, device="cuda")
+        _need(I, (B, self.ny, self.nx), torch.float32, "I")
+        self._call(fn, ctypes.byref(ls), ctypes.byref(cs), _ptr(Psi), B, _ptr(I), None, None, float(w),
+                   1 if accumulate else 0)
+        return I
+
+    def pctf_image(self, lens, coh, Psi, full_wave=False, w=1.0, out=None, accumulate=False):
+        """Quasi-coherent image; coh: dict dsf_sigma (A), ssf_beta (mrad), mode."""
+        return self._coherent_image("em_pctf_image", lens, coh, Psi, full_wave, w, out, accumulate)
+
+    def tcc_image(self, lens, coh, Psi, full_wave=False, w=1.0, out=None, accumulate=False):
+        """Image averaged over defocus / tilt Gauss-Hermite nodes; coh adds
+        dsf_npoints and ssf_npoints."""
+        return self._coherent_image("em_tcc_image", lens, coh, Psi, full_wave, w, out, accumulate)
+
     def diffraction(self, Psi, w=1.0, out=None, accumulate=False):
         torch = _torch()
         B = Psi.shape[0]

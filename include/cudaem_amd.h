@@ -140,6 +140,21 @@ em_status em_set_element_coefs(em_plan plan, int Z, int nk, const double* a, con
 /* Spline nodes: nR nodes log-spaced in R over [r_min, r_max], cosine taper
  * beyond tap_frac * r_max (defaults 128, 0.005, 3.0, 0.85). */
 em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max, double tap_frac);
+/* Partial coherence of the objective lens (Lens_Dev dsf / ssf fields,
+ * lens_dev.cuh:27-34).  dsf_sigma: standard deviation of the Gaussian defocus
+ * spread (A); ssf_beta: e^-1 half-width semi-angle of the Gaussian source
+ * (mrad), i.e. tilts q ~ exp(-|q|^2 / q0^2) with q0 = sin(beta) / lambda;
+ * *_npoints: Gauss-Hermite points of the TCC integrals (defocus; per tilt
+ * axis); mode: which spreads apply (eTemporal_Spatial_Incoh). */
+enum { EM_COH_TEMPORAL_SPATIAL = 1, EM_COH_TEMPORAL = 2, EM_COH_SPATIAL = 3 };
+typedef struct em_coherence {
+  double dsf_sigma;
+  int dsf_npoints;
+  double ssf_beta;
+  int ssf_npoints;
+  int mode;
+} em_coherence;
+
 /* 3D potential parameterisations for fine slicing (Vr_dVrir_*,
  * basic_functions_dev.cuh:104-214; the reference's ePotential_Type values are
  * in a missing header, these numbers are the build's). */
@@ -187,6 +202,9 @@ em_status em_spline_tables(em_plan plan, int max_types, int* Zs, int* ntypes, in
  * Returns EM_ERR_INVALID when more than max_slices slices would be made. */
 em_status em_host_slicing(long long n, const double* z_sorted, double dz, int border, int max_slices,
                           int* n_slices, double* bounds, long long* iatom_start, long long* iatom_end);
+/* Quadrature nodes the engine uses, host only: kind 0 = Gauss-Legendre on
+ * [-1/2, 1/2] (fine slicing), kind 1 = Gauss-Hermite for exp(-t^2) (TCC). */
+em_status em_host_quadrature(int kind, int n, double* x, double* w);
 /* Fine-sliced slicing (slice_generator.hpp:88-93, 108-128), host only: as
  * em_host_slicing over [z_min - r_int, z_max + r_int] with atom windows
  * [min(z_mid - r_int, z_start) - eps, max(z_mid + r_int, z_end) + eps]. */
@@ -248,6 +266,26 @@ em_status em_detector_sums(em_plan plan, const void* Psi_dev, int nprobe, int nd
 em_status em_ctf_image(em_plan plan, const em_lens* obj, const void* Psi_dev, int nprobe,
                        float* I_dev, float* re_dev, float* im_dev, float w, int accumulate);
 /* |Psi|^2 quadrant-shifted (ED / CBED output, single_simulator.cuh:103-110, 189). */
+/* Partially coherent image in the quasi-coherent approximation
+ * (Lens_Modulation::PCTF_TEM, lens_modulation.cuh:91-127; apply_PCTF_kernel,
+ * device_kernels.cuh:504-518): Psi * A(g) * E(g) * e^{i chi(g)}, then IFFT
+ * and |.|^2 (or re / im), with the envelope
+ *   u = 1 + 2 (pi beta sigma)^2 g^2,
+ *   E = exp(-[(pi beta)^2 g^2 (C30 lambda^2 g^2 + C10)^2
+ *            + (pi lambda sigma g^2)^2 / 2] / u) / sqrt(u).
+ * The reference's envelope writes (C30 lambda^2 g^2 - C10), the opposite
+ * defocus sign to its own chi, and takes its defocus half-width for sigma;
+ * this form is the Gaussian average of two-beam interference (to first order
+ * in the tilt) and agrees with em_tcc_image.  Arguments as em_ctf_image. */
+em_status em_pctf_image(em_plan plan, const em_lens* obj, const em_coherence* coh, const void* Psi,
+                        int nprobe, float* image, float* re, float* im, float w, int accumulate);
+/* Partially coherent image by the transmission cross coefficient
+ * (Lens_Modulation::TCC_TEM, lens_modulation.cuh:129-222): the weighted sum
+ * of coherent CTF images over Gauss-Hermite nodes, defocus
+ * C10 + sqrt(2) sigma t_j (weight w_j / sqrt(pi)) and tilt
+ * q0 (t_i, t_k) added to g (weight w_i w_k / pi).  1 to 64 points each. */
+em_status em_tcc_image(em_plan plan, const em_lens* obj, const em_coherence* coh, const void* Psi,
+                       int nprobe, float* image, float* re, float* im, float w, int accumulate);
 em_status em_diffraction(em_plan plan, const void* Psi_dev, int nprobe, float* I_dev, float w,
                          int accumulate);
 

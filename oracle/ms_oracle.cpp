@@ -754,9 +754,177 @@ using namespace orc;
 // ===========================================================================
 // C ABI for ctypes (tests and the CPU baseline).  All arrays are host arrays.
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Partial coherence of the objective lens (Lens_Modulation::PCTF_TEM /
+// TCC_TEM, lens_modulation.cuh:91-222; apply_PCTF_kernel,
+// device_kernels.cuh:504-518; incoherent_phase_shift,
+// basic_functions_dev.cuh:236-266).
+//
+// Conventions fixed by the build (the reference's constants live in missing
+// headers): sigma = standard deviation of the Gaussian defocus spread (A);
+// beta = e^-1 half-width semi-angle of the Gaussian source (rad), i.e. tilt
+// density ~ exp(-|q|^2 / q0^2) with q0 = sin(beta) / lambda.
+//   PCTF (quasi-coherent, linear imaging): Psi(g) A(g) E(g) e^{i chi(g)} with
+//     u = 1 + 2 (pi beta sigma)^2 g^2,
+//     E = exp(-[(pi beta)^2 g^2 (c30 lambda^2 g^2 + c10)^2
+//              + (pi lambda sigma g^2)^2 / 2] / u) / sqrt(u),
+//   which is the exact Gaussian average of the two-beam interference term to
+//   first order in the tilt (the reference writes (c30 lambda^2 g^2 - c10),
+//   the opposite defocus sign to its own chi, and its defocus width as the
+//   e^-1 half-width; both are made consistent with the TCC here).
+//   TCC: sum over Gauss-Hermite nodes of coherent CTF images,
+//     defocus c10 + sqrt(2) sigma t_j, weight w_j / sqrt(pi);
+//     tilt q = q0 (t_i, t_k), weight w_i w_k / pi;
+//   mode 1 = both, 2 = temporal only, 3 = spatial only.
+// ---------------------------------------------------------------------------
+// Gauss-Hermite nodes / weights for weight exp(-t^2): Newton on the
+// orthonormal Hermite recurrence (textbook), ascending nodes.
+void gauss_hermite(int n, double* x, double* w) {
+  const double pim4 = 0.7511255444649425;  // pi^(-1/4)
+  const int m = (n + 1) / 2;
+  double z = 0.0;
+  for (int i = 0; i < m; ++i) {
+    if (i == 0) z = std::sqrt(2.0 * n + 1.0) - 1.85575 * std::pow(2.0 * n + 1.0, -0.16667);
+    else if (i == 1) z -= 1.14 * std::pow((double)n, 0.426) / z;
+    else if (i == 2) z = 1.86 * z - 0.86 * x[0];
+    else if (i == 3) z = 1.91 * z - 0.91 * x[1];
+    else z = 2.0 * z - x[i - 2];
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = pim4, p2 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = z * std::sqrt(2.0 / (j + 1)) * p2 - std::sqrt((double)j / (j + 1)) * p3;
+      }
+      pp = std::sqrt(2.0 * n) * p2;
+      const double dz = p1 / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-15) break;
+    }
+    x[i] = z;  // descending positive nodes for now
+    w[i] = 2.0 / (pp * pp);
+  }
+  // mirror into ascending order
+  std::vector<double> xs(n), ws(n);
+  for (int i = 0; i < m; ++i) {
+    xs[n - 1 - i] = x[i];
+    ws[n - 1 - i] = w[i];
+    xs[i] = -x[i];
+    ws[i] = w[i];
+  }
+  if (n % 2 == 1) xs[n / 2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    x[i] = xs[i];
+    w[i] = ws[i];
+  }
+}
+
+double pctf_envelope(double g2, double lambda, double c10, double c30, double sigma, double beta) {
+  const double c = kPi * beta * sigma;
+  const double u = 1.0 + 2.0 * c * c * g2;
+  const double tc = kPi * lambda * sigma * g2;
+  const double temp = 0.5 * tc * tc;
+  const double sc = kPi * beta * (c30 * lambda * lambda * g2 + c10);
+  const double spa = sc * sc * g2;
+  return std::exp(-(spa + temp) / u) / std::sqrt(u);
+}
+
+// Coherent CTF image with the lens shifted by the tilt (gxu, gyu) and the
+// defocus by dc10 (apply_CTF_kernel, device_kernels.cuh:445-468), |.|^2 or
+// (re, im) scaled by wt and accumulated.
+static void ctf_accumulate(const Grid& g, const double* lens, double lambda, double dc10, double gxu,
+                           double gyu, const cd* in, double wt, double* inten, double* re, double* im) {
+  std::vector<double> lp(lens, lens + L_COUNT);
+  lp[L_C10] += dc10;
+  Lens l(lp.data(), lambda, 0.0);
+  const long long N = g.n();
+  std::vector<cd> w(N);
+  for (int iy = 0; iy < g.ny; ++iy)
+    for (int ix = 0; ix < g.nx; ++ix) {
+      const double gx = g.gx(ix) + gxu, gy = g.gy(iy) + gyu, g2 = gx * gx + gy * gy;
+      const long long i = (long long)iy * g.nx + ix;
+      if (l.g2_min <= g2 && g2 < l.g2_max) {
+        const double ph = l.chi(gx, gy);
+        w[i] = in[i] * cd(std::cos(ph), std::sin(ph));
+      } else {
+        w[i] = 0.0;
+      }
+    }
+  fft2(w.data(), g.nx, g.ny, +1);
+  for (long long i = 0; i < N; ++i) {
+    if (inten) inten[i] += wt * std::norm(w[i]);
+    if (re) re[i] += wt * w[i].real();
+    if (im) im[i] += wt * w[i].imag();
+  }
+}
+
 extern "C" {
 
 int orc_version(void) { return 1; }
+
+void orc_gauss_hermite(int n, double* x, double* w) { gauss_hermite(n, x, w); }
+
+// PCTF image (PCTF_TEM): sigma (A), beta (mrad), mode as above.
+void orc_pctf_image(int nx, int ny, double lx, double ly, double lambda, const double* lens, double sigma,
+                    double beta_mrad, int mode, const double* Psi, double* inten, double* re, double* im) {
+  Grid g(nx, ny, lx, ly, 0.0, 0);
+  Lens l(lens, lambda, 0.0);
+  const double s = (mode == 3) ? 0.0 : sigma, b = (mode == 2) ? 0.0 : beta_mrad * 1e-3;
+  const long long N = g.n();
+  std::vector<cd> w(N);
+  const cd* in = reinterpret_cast<const cd*>(Psi);
+  for (int iy = 0; iy < ny; ++iy)
+    for (int ix = 0; ix < nx; ++ix) {
+      const double gx = g.gx(ix), gy = g.gy(iy), g2 = gx * gx + gy * gy;
+      const long long i = (long long)iy * nx + ix;
+      if (l.g2_min <= g2 && g2 < l.g2_max) {
+        const double ph = l.chi(gx, gy);
+        const double e = pctf_envelope(g2, lambda, lens[L_C10], lens[L_C30], s, b);
+        w[i] = in[i] * std::polar(e, ph);
+      } else {
+        w[i] = 0.0;
+      }
+    }
+  fft2(w.data(), nx, ny, +1);
+  for (long long i = 0; i < N; ++i) {
+    if (inten) inten[i] = std::norm(w[i]);
+    if (re) re[i] = w[i].real();
+    if (im) im[i] = w[i].imag();
+  }
+}
+
+// TCC image (TCC_TEM): nt / ns Gauss-Hermite points of the defocus and of
+// each tilt axis.
+void orc_tcc_image(int nx, int ny, double lx, double ly, double lambda, const double* lens, double sigma,
+                   int nt, double beta_mrad, int ns, int mode, const double* Psi, double* inten, double* re,
+                   double* im) {
+  Grid g(nx, ny, lx, ly, 0.0, 0);
+  const long long N = g.n();
+  for (long long i = 0; i < N; ++i) {
+    if (inten) inten[i] = 0.0;
+    if (re) re[i] = 0.0;
+    if (im) im[i] = 0.0;
+  }
+  const cd* in = reinterpret_cast<const cd*>(Psi);
+  std::vector<double> tx(std::max(nt, 1)), tw(std::max(nt, 1)), sx(std::max(ns, 1)), sw(std::max(ns, 1));
+  const bool temporal = mode != 3, spatial = mode != 2;
+  const int NT = temporal ? nt : 1, NS = spatial ? ns : 1;
+  if (temporal) gauss_hermite(nt, tx.data(), tw.data()); else { tx[0] = 0.0; tw[0] = std::sqrt(kPi); }
+  if (spatial) gauss_hermite(ns, sx.data(), sw.data()); else { sx[0] = 0.0; sw[0] = std::sqrt(kPi); }
+  const double q0 = std::sin(beta_mrad * 1e-3) / lambda;
+  for (int a = 0; a < NS; ++a)
+    for (int b = 0; b < NS; ++b) {
+      if (!spatial && (a > 0 || b > 0)) continue;
+      const double ws = spatial ? sw[a] * sw[b] / kPi : 1.0;
+      const double qx = spatial ? q0 * sx[a] : 0.0, qy = spatial ? q0 * sx[b] : 0.0;
+      for (int j = 0; j < NT; ++j) {
+        const double wt = temporal ? tw[j] / std::sqrt(kPi) : 1.0;
+        const double df = temporal ? std::sqrt(2.0) * sigma * tx[j] : 0.0;
+        ctf_accumulate(g, lens, lambda, df, qx, qy, in, ws * wt, inten, re, im);
+      }
+    }
+}
 
 void orc_constants(double e_kv, double* lam, double* gam, double* sig) {
   *lam = wavelength(e_kv);

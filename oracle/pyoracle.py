@@ -70,6 +70,9 @@ def lib():
                                _I, _PD, _PD, _D, _PD, _I]
         L.orc_stem.restype = _I
         L.orc_adjoint.argtypes = [_I, _I, _PD, _I, _PD, _I, _PD, _PD, _PD, _PD]
+        L.orc_gauss_hermite.argtypes = [_I, _PD, _PD]
+        L.orc_pctf_image.argtypes = [_I, _I, _D, _D, _D, _PD, _D, _D, _I, _PD, _PD, _PD, _PD]
+        L.orc_tcc_image.argtypes = [_I, _I, _D, _D, _D, _PD, _D, _I, _D, _I, _I, _PD, _PD, _PD, _PD]
         L.orc_slicing_fine.argtypes = [_I, _PD, _D, _I, _D, _D, _I, _PD, _PLL, _PLL]
         L.orc_slicing_fine.restype = _I
         L.orc_vr3d.argtypes = [_I, _I, _PD, _PD, _PD, _PD, _PD]
@@ -350,6 +353,36 @@ def ctf_image(Psi, lx, ly, lam, lens):
     lib().orc_ctf_image(nx, ny, lx, ly, lam, _pd(la), Psi.view(np.float64).ctypes.data_as(_PD),
                         _pd(I), _pd(re), _pd(im))
     return I, re + 1j * im
+
+
+# --------------------------------------------------------------------------- partial coherence
+COH_BOTH, COH_TEMPORAL, COH_SPATIAL = 1, 2, 3
+
+
+def gauss_hermite(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauss_hermite(n, _pd(x), _pd(w))
+    return x, w
+
+
+def pctf_image(Psi, lx, ly, lam, lens, sigma, beta_mrad, mode=COH_BOTH, full=False):
+    """Quasi-coherent image: returns |.|^2, or (re, im) with full=True."""
+    a = _c128(Psi)
+    ny, nx = a.shape
+    I, re, im = np.zeros((ny, nx)), np.zeros((ny, nx)), np.zeros((ny, nx))
+    lib().orc_pctf_image(nx, ny, lx, ly, lam, _pd(lens_array(lens)), sigma, beta_mrad, mode,
+                         a.view(np.float64).ctypes.data_as(_PD), _pd(I), _pd(re), _pd(im))
+    return (re, im) if full else I
+
+
+def tcc_image(Psi, lx, ly, lam, lens, sigma, nt, beta_mrad, ns, mode=COH_BOTH, full=False):
+    a = _c128(Psi)
+    ny, nx = a.shape
+    I, re, im = np.zeros((ny, nx)), np.zeros((ny, nx)), np.zeros((ny, nx))
+    lib().orc_tcc_image(nx, ny, lx, ly, lam, _pd(lens_array(lens)), sigma, nt, beta_mrad, ns, mode,
+                        a.view(np.float64).ctypes.data_as(_PD), _pd(I), _pd(re), _pd(im))
+    return (re, im) if full else I
 
 
 def thickness(P, V, psi, lx, ly, g2min, g2max, stride=1):

@@ -97,6 +97,10 @@ def main():
     out["vr_R"] = Rs
     for Z in (8, 22, 38):
         out[f"vr_{Z}"] = np.array([moliere_projected(Z, R) for R in Rs])
+    for n in (1, 2, 12, 24):
+        x, w = np.polynomial.hermite.hermgauss(n)
+        out[f"gh_x_{n}"] = x
+        out[f"gh_w_{n}"] = w
     for n in (7, 64, 128):
         x, w = np.polynomial.legendre.leggauss(n)
         out[f"gl_x_{n}"] = x / 2

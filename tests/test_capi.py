@@ -127,6 +127,16 @@ def test_host_slicing_fine_matches_oracle(C, border):
             assert np.array_equal(b[:S + 1], bo) and np.array_equal(i0[:S], i0o) and np.array_equal(i1[:S], i1o)
 
 
+@pytest.mark.parametrize("kind,n", [(0, 7), (0, 128), (1, 1), (1, 12), (1, 24)])
+def test_host_quadrature_matches_oracle(C, kind, n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert C.lib().em_host_quadrature(kind, n, x.ctypes.data_as(C._PD), w.ctypes.data_as(C._PD)) == 0
+    xo, wo = O.gauss_legendre(n) if kind == 0 else O.gauss_hermite(n)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(w, wo, rtol=1e-13, atol=1e-16)
+
+
 def test_host_slicing_rejects_unsorted(C):
     z = np.array([1.0, 0.5])
     ns = ctypes.c_int()

@@ -139,6 +139,36 @@ def test_fine_sliced_hrtem_matches_oracle(em):
     assert rel_l2(Psi, Psio) < TOL_WAVE
 
 
+def test_partial_coherence_images_match_oracle(em):
+    """PCTF (quasi-coherent envelope) and TCC (Gauss-Hermite average of
+    coherent images over defocus and tilt) of one exit wave, against the
+    oracle on the same wave."""
+    import cudaem_amd.specimen as sp
+    import torch
+    at = sp.fcc_cu(4, 4, 3)
+    n, dz, e0 = 128, 1.8075, 200.0
+    obj = {"c_30": 1.0e7, "c_10": -600.0, "outer_aper_mrad": 30.0}
+    coh = {"dsf_sigma": 35.0, "dsf_npoints": 5, "ssf_beta": 0.4, "ssf_npoints": 4, "mode": 1}
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MAX)
+        V = p.potential_slices()
+        psi = p.plane_wave(1)
+        p.multislice_forward(psi, V=V)
+        Psi = _np(psi)[0].astype(np.complex128)
+        Ip = _np(p.pctf_image(obj, coh, psi))[0]
+        re, im = p.pctf_image(obj, dict(coh, mode=2), psi, full_wave=True)
+        It = _np(p.tcc_image(obj, coh, psi))[0]
+        Its = _np(p.tcc_image(obj, dict(coh, mode=3), psi))[0]
+        lam = p.lam
+        torch.cuda.synchronize()
+    L = at["lx"]
+    assert rel_l2(Ip, O.pctf_image(Psi, L, L, lam, obj, 35.0, 0.4)) < TOL_INT
+    ro, io = O.pctf_image(Psi, L, L, lam, obj, 35.0, 0.4, O.COH_TEMPORAL, full=True)
+    assert rel_l2(_np(re)[0] + 1j * _np(im)[0], ro + 1j * io) < TOL_WAVE
+    assert rel_l2(It, O.tcc_image(Psi, L, L, lam, obj, 35.0, 5, 0.4, 4)) < TOL_INT
+    assert rel_l2(Its, O.tcc_image(Psi, L, L, lam, obj, 35.0, 5, 0.4, 4, O.COH_SPATIAL)) < TOL_INT
+
+
 def test_spline_tables_match_oracle(em):
     import cudaem_amd.specimen as sp
     at = sp.srtio3(2, 2, 2)

@@ -530,3 +530,95 @@ def test_potential_fine_brute_force_and_slice_sum():
         ref += O.potential_fine(nx, ny, lx, ly, zs, ze, a, a, x, y, z, occ, types, pts, cls, cnls, 1.0,
                                 nq=2048)
     np.testing.assert_allclose(total, ref, rtol=0, atol=1e-8 * abs(ref).max())
+
+
+# ----------------------------------------------------------------- partial coherence
+@pytest.mark.parametrize("n", [1, 2, 12, 24])
+def test_gauss_hermite_matches_numpy(n):
+    x, w = O.gauss_hermite(n)
+    np.testing.assert_allclose(x, GOLD[f"gh_x_{n}"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(w, GOLD[f"gh_w_{n}"], rtol=1e-12, atol=1e-16)
+
+
+def _two_beam(n=64, L=20.0, h=10, b=0.3):
+    Psi = np.zeros((n, n), complex)
+    Psi[0, 0] = n * n
+    Psi[0, h] = b * n * n
+    return Psi, h / L
+
+
+def _cross(I, h):
+    n = I.shape[0]
+    return np.fft.fft2(I)[0, h] / (n * n)
+
+
+LENS_PC = {"c_30": 1e7, "c_10": -500.0}
+
+
+def test_partial_coherence_reduces_to_the_ctf():
+    """No spread: the PCTF is the CTF image; one node at zero with weight 1
+    makes the TCC the CTF image."""
+    rng = np.random.default_rng(3)
+    n, L = 32, 12.0
+    lam = O.constants(300.0)[0]
+    Psi = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    lens = dict(LENS_PC, outer_aper_mrad=30.0)
+    I, _ = O.ctf_image(Psi, L, L, lam, lens)
+    np.testing.assert_allclose(O.pctf_image(Psi, L, L, lam, lens, 0.0, 0.0), I, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(O.tcc_image(Psi, L, L, lam, lens, 25.0, 1, 0.4, 1), I, rtol=1e-12, atol=0)
+
+
+def test_two_beam_temporal_damping_is_the_gaussian_characteristic_function():
+    """A Gaussian defocus spread of standard deviation sigma damps the
+    interference of beams 0 and g by exp(-(pi lambda sigma g^2)^2 / 2), in
+    the PCTF envelope and in the TCC average alike."""
+    lam = O.constants(300.0)[0]
+    Psi, g = _two_beam()
+    sigma = 40.0
+    c = _cross(O.pctf_image(Psi, 20.0, 20.0, lam, LENS_PC, 0.0, 0.0), 10)
+    p = _cross(O.pctf_image(Psi, 20.0, 20.0, lam, LENS_PC, sigma, 0.0, O.COH_TEMPORAL), 10)
+    t = _cross(O.tcc_image(Psi, 20.0, 20.0, lam, LENS_PC, sigma, 30, 0.0, 1, O.COH_TEMPORAL), 10)
+    exact = np.exp(-0.5 * (np.pi * lam * sigma * g * g) ** 2)
+    assert abs(abs(p) / abs(c) - exact) < 1e-12
+    assert abs(abs(t) / abs(c) - exact) < 1e-12
+    assert abs(np.angle(t / c)) < 1e-12
+
+
+def test_two_beam_spatial_damping_follows_the_chi_gradient():
+    """Tilt spread: the exact TCC average agrees with the PCTF envelope
+    exp(-(pi beta)^2 g^2 (c30 lambda^2 g^2 + c10)^2) to second order in the
+    tilt, and rules out the opposite defocus sign."""
+    lam = O.constants(300.0)[0]
+    Psi, g = _two_beam()
+    beta = 0.3
+    c = abs(_cross(O.pctf_image(Psi, 20.0, 20.0, lam, LENS_PC, 0.0, 0.0), 10))
+    p = abs(_cross(O.pctf_image(Psi, 20.0, 20.0, lam, LENS_PC, 0.0, beta, O.COH_SPATIAL), 10)) / c
+    t = abs(_cross(O.tcc_image(Psi, 20.0, 20.0, lam, LENS_PC, 0.0, 1, beta, 20, O.COH_SPATIAL), 10)) / c
+    b = beta * 1e-3
+    plus = np.exp(-(np.pi * b) ** 2 * g * g * (1e7 * lam ** 2 * g * g - 500.0) ** 2)
+    minus = np.exp(-(np.pi * b) ** 2 * g * g * (1e7 * lam ** 2 * g * g + 500.0) ** 2)
+    assert abs(p - plus) < 1e-12
+    assert abs(t - p) < 2e-3
+    assert abs(t - minus) > 0.2
+    # joint spreads: the coupled envelope (the 1/u terms) against the 2D + 1D average
+    pj = abs(_cross(O.pctf_image(Psi, 20.0, 20.0, lam, LENS_PC, 40.0, beta), 10)) / c
+    tj = abs(_cross(O.tcc_image(Psi, 20.0, 20.0, lam, LENS_PC, 40.0, 20, beta, 16), 10)) / c
+    assert abs(tj - pj) < 2e-3
+
+
+def test_tcc_of_a_weak_phase_object_is_close_to_the_pctf():
+    """For a weak-phase object the image is linear in the scattered wave, the
+    regime the quasi-coherent PCTF describes: the two images agree to a
+    small fraction of the contrast."""
+    rng = np.random.default_rng(9)
+    n, L = 64, 16.0
+    lam = O.constants(300.0)[0]
+    phase = 0.02 * np.real(np.fft.ifft2(np.fft.fft2(rng.standard_normal((n, n))) *
+                                        (np.add.outer(np.fft.fftfreq(n) ** 2, np.fft.fftfreq(n) ** 2) < 0.05)))
+    Psi = np.fft.fft2(np.exp(1j * phase))
+    lens = dict(LENS_PC, outer_aper_mrad=25.0)
+    p = O.pctf_image(Psi, L, L, lam, lens, 30.0, 0.2)
+    t = O.tcc_image(Psi, L, L, lam, lens, 30.0, 12, 0.2, 10)
+    contrast = np.std(t)
+    assert contrast > 1e-3 * np.mean(t)
+    assert np.std(p - t) < 0.02 * contrast
