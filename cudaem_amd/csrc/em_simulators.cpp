@@ -195,7 +195,9 @@ void multislice_params<T>::validate_parameters() const {
   if (simulation_type == eTEMST_STEM && (STEM_col <= 0 || STEM_row <= 0 || detector.size() == 0))
     throw simulation_error("STEM needs a scan and detectors");
   if (frozen_phonon_enable() && pn_nconf <= 0) throw simulation_error("pn_nconf");
-  if (illumination_model != eIM_Coherent) throw simulation_error("only coherent illumination is built");
+  if (illumination_model != eIM_Coherent && illumination_model != eIM_Partial_Coherent &&
+      illumination_model != eIM_Trans_Cross_Coef)
+    throw simulation_error("illumination model");
 }
 
 template <class T>
@@ -267,10 +269,22 @@ void run_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T>* o)
     }
     ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr, 1), plan.p, "em_multislice_forward");
     if (m.simulation_type == eTEMST_HRTEM) {
-      if (full)
-        ok(em_ctf_image(plan.p, &obj, psi.p, 1, nullptr, a.as<float>(), b.as<float>(), w, 1), plan.p, "em_ctf_image");
+      // Lens_Modulation::wave_modulation's illumination switch (lens_modulation.cuh:52-73)
+      float* I = full ? nullptr : a.as<float>();
+      float* re = full ? a.as<float>() : nullptr;
+      float* im = full ? b.as<float>() : nullptr;
+      em_coherence coh;
+      coh.dsf_sigma = m.obj_lens.dsf_sigma;
+      coh.dsf_npoints = m.obj_lens.dsf_npoints;
+      coh.ssf_beta = m.obj_lens.ssf_beta;
+      coh.ssf_npoints = m.obj_lens.ssf_npoints;
+      coh.mode = (int)m.temp_spat_incoh;
+      if (m.illumination_model == eIM_Partial_Coherent)
+        ok(em_pctf_image(plan.p, &obj, &coh, psi.p, 1, I, re, im, w, 1), plan.p, "em_pctf_image");
+      else if (m.illumination_model == eIM_Trans_Cross_Coef)
+        ok(em_tcc_image(plan.p, &obj, &coh, psi.p, 1, I, re, im, w, 1), plan.p, "em_tcc_image");
       else
-        ok(em_ctf_image(plan.p, &obj, psi.p, 1, a.as<float>(), nullptr, nullptr, w, 1), plan.p, "em_ctf_image");
+        ok(em_ctf_image(plan.p, &obj, psi.p, 1, I, re, im, w, 1), plan.p, "em_ctf_image");
     } else if (full) {
       hip_ok(em::launch_split(psi.as<float2>(), a.as<float>(), b.as<float>(), (long long)N, w, 1, nullptr), "split");
     } else {

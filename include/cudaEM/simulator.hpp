@@ -28,6 +28,7 @@ enum eTEM_Sim_Type { eTEMST_HRTEM = 1, eTEMST_ED = 2, eTEMST_CBED = 3, eTEMST_ST
 enum eIncident_Wave_Type { eIWT_Plane_Wave = 1, eIWT_Convergent_Wave = 2 };
 enum ePhonon_Model { ePM_Still_Atom = 1, ePM_Absorptive = 2, ePM_Frozen_Phonon = 3 };
 enum eIllumination_Model { eIM_Coherent = 1, eIM_Partial_Coherent = 2, eIM_Trans_Cross_Coef = 3 };
+enum eTemporal_Spatial_Incoh { eTSI_Temporal_Spatial = 1, eTSI_Temporal = 2, eTSI_Spatial = 3 };
 enum eScanning_Type { eTile_scanning = 1, eGeneral_scanning = 2 };
 enum eData_Record { eIntensity = 1, eWave = 2 };
 enum eMatch_Border { eMB_Min = 0, eMB_Max = 1, eMB_Middle = 2 };
@@ -71,6 +72,13 @@ struct Lens {
   T c_41 = 0, phi_41 = 0, c_43 = 0, phi_43 = 0, c_45 = 0, phi_45 = 0;
   T c_50 = 0, c_52 = 0, phi_52 = 0, c_54 = 0, phi_54 = 0, c_56 = 0, phi_56 = 0;
   T inner_aper_ang = 0, outer_aper_ang = 0;  // mrad
+  // partial coherence (lens_dev.cuh:27-34): standard deviation of the
+  // defocus spread (A) and e^-1 half-width semi-angle of the Gaussian source
+  // (mrad), with the Gauss-Hermite points of the TCC integrals
+  T dsf_sigma = 0;
+  int dsf_npoints = 1;
+  T ssf_beta = 0;
+  int ssf_npoints = 1;
 };
 
 // Annular detectors; angles in mrad (wave_function.cuh:135-162 squares the
@@ -105,6 +113,7 @@ struct multislice_params {
   eIncident_Wave_Type iw_type = eIWT_Plane_Wave;
   T incident_wave_x_pos = 0, incident_wave_y_pos = 0;
   eIllumination_Model illumination_model = eIM_Coherent;
+  eTemporal_Spatial_Incoh temp_spat_incoh = eTSI_Temporal_Spatial;  // spreads of partial coherence
   // specimen
   Atom_Coord_Data<T> atoms;
   // fine slicing (multislice_params_dev.cuh:23-24): each element's 3D

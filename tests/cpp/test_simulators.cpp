@@ -81,6 +81,22 @@ int main(int argc, char** argv) {
       write(out + "/hrtem.f32", o.image_intensity);
       std::printf("hrtem %zu\n", o.image_intensity.size());
     }
+    for (int model = 2; model <= 3; ++model) {  // HRTEM, partial coherence: PCTF, TCC
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_HRTEM;
+      p.illumination_model = (eIllumination_Model)model;
+      p.obj_lens.c_30 = 1e7f;
+      p.obj_lens.c_10 = -500.f;
+      p.obj_lens.outer_aper_ang = 30.f;
+      p.obj_lens.dsf_sigma = 40.f;
+      p.obj_lens.dsf_npoints = 5;
+      p.obj_lens.ssf_beta = 0.3f;
+      p.obj_lens.ssf_npoints = 3;
+      output_Multi_Data<float> o;
+      sim.executionTEM(p, &o);
+      write(out + (model == 2 ? "/hrtem_pctf.f32" : "/hrtem_tcc.f32"), o.image_intensity);
+      std::printf("hrtem partial coherence %d %zu\n", model, o.image_intensity.size());
+    }
     {  // ED, full wave
       multislice_params<float> p = base(128, 3, 4);
       p.simulation_type = eTEMST_ED;

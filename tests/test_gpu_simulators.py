@@ -64,6 +64,22 @@ def test_cpp_hrtem(outputs):
     assert rel_l2(outputs("hrtem.f32").reshape(N, N), I) < 1e-3
 
 
+@pytest.mark.parametrize("model", ["pctf", "tcc"])
+def test_cpp_hrtem_partial_coherence(outputs, model):
+    """illumination_model eIM_Partial_Coherent / eIM_Trans_Cross_Coef with
+    the objective lens's defocus and source spreads."""
+    at = atoms_f32()
+    o = OracleSpecimen(at, N, N, L, L, DZ, E0, O.BORDER_MAX)
+    Psi = O.multislice(o.propagator(), np.ones((N, N), complex), V=o.potentials())
+    lens = {"c_30": 1e7, "c_10": -500.0, "outer_aper_mrad": 30.0}
+    sig, beta = float(np.float32(40.0)), float(np.float32(0.3))
+    if model == "pctf":
+        I = O.pctf_image(Psi, L, L, o.lam, lens, sig, beta)
+    else:
+        I = O.tcc_image(Psi, L, L, o.lam, lens, sig, 5, beta, 3)
+    assert rel_l2(outputs(f"hrtem_{model}.f32").reshape(N, N), I) < 1e-3
+
+
 def test_cpp_ed_full_wave(outputs):
     at = atoms_f32()
     o = OracleSpecimen(at, N, N, L, L, DZ, E0, O.BORDER_MIN)
