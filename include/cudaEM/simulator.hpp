@@ -18,6 +18,9 @@
 // multislice_params_dev.cuh:92-97, and averages on the host).
 // Errors throw cudaEM::simulation_error (the reference prints and continues).
 #pragma once
+#include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -237,6 +240,57 @@ void run_thickness_simu(multislice_params<T>* p, int GPU_ID, output_Multi_Data<T
 template <class T>
 void vibrate_atom_positions(const Atom_Coord_Data<T>& in, Atom_Coord_Data<T>& out, unsigned long long seed,
                             int iconf, T sigma, const bool dims[3]);
+
+// Raw result files (the reference's rawdata_io.hpp is missing): the values
+// of v in native byte order, no header.
+template <class T>
+void write_raw_data(const std::string& path, const std::vector<T>& v) {
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw simulation_error("cannot write " + path);
+  const size_t n = v.empty() ? 0 : std::fwrite(v.data(), sizeof(T), v.size(), f);
+  const bool closed = std::fclose(f) == 0;
+  if (n != v.size() || !closed) throw simulation_error("short write to " + path);
+}
+
+template <class T>
+std::vector<T> read_raw_data(const std::string& path) {
+  std::FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) throw simulation_error("cannot read " + path);
+  std::vector<T> v;
+  T buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, sizeof(T), 4096, f)) > 0) v.insert(v.end(), buf, buf + n);
+  const bool bad = std::ferror(f) != 0;
+  std::fclose(f);
+  if (bad) throw simulation_error("read error in " + path);
+  return v;
+}
+
+// Atom list as text (the reference's iolib.hpp is missing): one atom per
+// line, "Z x y z [occ]" with positions in Angstrom; blank lines and text
+// after '#' are ignored.  x_l / y_l are left for the caller to set.
+template <class T>
+Atom_Coord_Data<T> read_atoms_text(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw simulation_error("cannot read " + path);
+  Atom_Coord_Data<T> at;
+  std::string line;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.erase(hash);
+    std::istringstream ss(line);
+    int Z;
+    double x, y, z, occ = 1.0;
+    if (!(ss >> Z)) continue;  // blank line
+    if (!(ss >> x >> y >> z)) throw simulation_error(path + ":" + std::to_string(lineno) + ": expected Z x y z");
+    if (!(ss >> occ)) occ = 1.0;
+    if (Z <= 0 || Z > 103) throw simulation_error(path + ":" + std::to_string(lineno) + ": atomic number");
+    at.push_back(Z, (T)x, (T)y, (T)z, (T)occ);
+  }
+  return at;
+}
 
 }  // namespace cudaEM
 

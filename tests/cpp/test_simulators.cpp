@@ -3,6 +3,7 @@
 // files for tests/test_gpu_simulators.py to compare with the oracle.
 //
 //   test_simulators vibrate <out_dir>   phonon displacements only (no GPU)
+//   test_simulators io <out_dir>        atoms.txt -> atoms.f32 through the I/O helpers (no GPU)
 //   test_simulators run <out_dir>       HRTEM, ED, CBED, STEM (tile and
 //                                       general scanning), thickness series
 #include <cstdio>
@@ -14,12 +15,7 @@
 
 using namespace cudaEM;
 
-static void write(const std::string& path, const std::vector<float>& v) {
-  FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) throw simulation_error("cannot write " + path);
-  std::fwrite(v.data(), sizeof(float), v.size(), f);
-  std::fclose(f);
-}
+static void write(const std::string& path, const std::vector<float>& v) { write_raw_data(path, v); }
 
 // SrTiO3 perovskite, n x n x nz cells (a = 3.905 A)
 static Atom_Coord_Data<float> srtio3(int n, int nz) {
@@ -56,6 +52,20 @@ int main(int argc, char** argv) {
   }
   const std::string mode = argv[1], out = argv[2];
   try {
+    if (mode == "io") {  // text atoms in, raw float32 out and back (no GPU)
+      const Atom_Coord_Data<float> at = read_atoms_text<float>(out + "/atoms.txt");
+      std::vector<float> r;
+      for (size_t i = 0; i < at.size(); ++i) {
+        r.push_back((float)at.Z[i]);
+        r.push_back(at.x[i]);
+        r.push_back(at.y[i]);
+        r.push_back(at.z[i]);
+        r.push_back(at.occ[i]);
+      }
+      write(out + "/atoms.f32", r);
+      if (read_raw_data<float>(out + "/atoms.f32") != r) throw simulation_error("raw round trip");
+      return 0;
+    }
     if (mode == "vibrate") {
       Atom_Coord_Data<double> at, v;
       for (int i = 0; i < 7; ++i) at.push_back(8, 0.5 * i, 1.0 * i, 0.25 * i);

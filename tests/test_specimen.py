@@ -65,6 +65,22 @@ def test_cpp_vibrate_matches_python(tmp_path):
     np.testing.assert_allclose(r, d, atol=1e-7)
 
 
+def test_cpp_atom_text_and_raw_io(tmp_path):
+    """read_atoms_text parses Z x y z [occ] lines with comments and blanks;
+    write_raw_data / read_raw_data round-trip float32 arrays."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_simulators")
+    if not os.path.exists(exe):
+        pytest.skip("C++ test program not built")
+    (tmp_path / "atoms.txt").write_text("# SrTiO3 fragment\n38 0 0 0\n\n22 1.9525 1.9525 1.9525 0.5  # half\n"
+                                        "8 1.9525 1.9525 0\n")
+    subprocess.run([exe, "io", str(tmp_path)], check=True)
+    r = np.fromfile(tmp_path / "atoms.f32", dtype=np.float32).reshape(-1, 5)
+    np.testing.assert_array_equal(r, np.array([[38, 0, 0, 0, 1], [22, 1.9525, 1.9525, 1.9525, 0.5],
+                                               [8, 1.9525, 1.9525, 0, 1]], dtype=np.float32))
+    (tmp_path / "atoms.txt").write_text("8 1.0 2.0\n")
+    assert subprocess.run([exe, "io", str(tmp_path)], capture_output=True).returncode != 0
+
+
 def test_raster_and_cut():
     x, y = sp.raster(4, 3, 8.0, 6.0)
     assert len(x) == 12 and x[1] == 2.0 and y[4] == 2.0
