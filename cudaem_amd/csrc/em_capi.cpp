@@ -776,6 +776,78 @@ em_status em_multislice_forward(em_plan p, const float* V, const void* t, int S,
   return EM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Backward pass of one probe group (nb probes at lam; their checkpoints at ck,
+// slice j of them at ck + j * ck_stride).  The gradient of each slice sums
+// this group's per-workgroup partials; `first` overwrites gt, later groups
+// add to it, so the result does not depend on scheduling.
+em_status adjoint_group(em_plan p, const float* V, const float2* tc, int S, const float2* ck, long long ck_stride,
+                        int every, float2* lam, int nb, float2* gtc, float2* gpsi0, bool first) {
+  const long long N = p->N();
+  const long long GN = (long long)nb * N;
+  const int pb = row_pb(p, nb);
+  const int nparts = (nb + pb - 1) / pb;
+  const bool direct = first && nparts == 1;  // the single partial is the gradient
+  if (!direct) EM_HIP(p, p->gpart.ensure(sizeof(float2) * N * nparts));
+  if (every > 1) {
+    // segment workspace: the recomputed waves entering each slice of a segment
+    EM_HIP(p, p->seg.ensure(sizeof(float2) * GN * every));
+    EM_HIP(p, p->segwave.ensure(sizeof(float2) * GN));
+  }
+  // lambda_{S-1} = FFT^H gPsi: column inverse now, row inverse inside the first backward row pass
+  EM_TRY(col(p, COL_PRE_IFFT | COL_STORE, col_args(p, lam, lam, nb)));
+  for (int k = S - 1; k >= 0; --k) {
+    const float2* ckk;
+    if (every == 1) {
+      ckk = ck + (long long)k * ck_stride;
+    } else {
+      const int j = k / every, s0 = j * every;
+      if (k == S - 1 || k % every == every - 1) {
+        // recompute the waves entering slices s0..k from checkpoint j
+        float2* sw = p->segwave.as<float2>();
+        float2* sg = p->seg.as<float2>();
+        EM_HIP(p, hipMemcpyAsync(sw, ck + (long long)j * ck_stride, sizeof(float2) * GN, hipMemcpyDeviceToDevice,
+                                 p->stream));
+        for (int s = s0; s <= k; ++s) {
+          RowArgs r = row_args_skip(p, sw, sw, nb, s > s0, s < k);
+          r.ckpt = sg + (long long)(s - s0) * GN;
+          unsigned m = ROW_CKPT_STORE | (s > s0 ? ROW_PRE_IFFT : 0u);
+          if (s < k) {
+            m |= ROW_POST_FFT | (V ? ROW_MUL_V : ROW_MUL_T);
+            if (V) r.V = V + (long long)s * N; else r.t = tc + (long long)s * N;
+          }
+          EM_TRY(row(p, m, r));
+          if (s < k) EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, sw, sw, nb)));
+        }
+      }
+      ckk = p->seg.as<float2>() + (long long)(k - s0) * GN;
+    }
+    RowArgs r = row_args_skip(p, lam, lam, nb, k < S - 1, k > 0);
+    r.ckpt = const_cast<float2*>(ckk);
+    r.gpart = direct ? gtc + (long long)k * N : p->gpart.as<float2>();
+    unsigned m = ROW_PRE_IFFT | ROW_GRAD;
+    if (k > 0 || gpsi0) {
+      m |= ROW_CONJ | (V ? ROW_MUL_V : ROW_MUL_T);
+      if (V) r.V = V + (long long)k * N; else r.t = tc + (long long)k * N;
+    }
+    if (k > 0) m |= ROW_POST_FFT;
+    else if (gpsi0) r.out = gpsi0;  // conj(t_0) lambda_0, real space
+    EM_TRY(row(p, m, r));
+    if (!direct)
+      EM_HIP(p, launch_grad_reduce(N, nparts, p->gpart.as<float2>(), gtc + (long long)k * N, first ? 0 : 1,
+                                   p->stream));
+    if (k > 0) EM_TRY(col(p, COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, lam, lam, nb)));
+  }
+  return EM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S, const void* ckpt, int every,
                                 void* gPsi, int nprobe, void* gt, float* gV, void* gpsi0) {
   if (!p || !ckpt || !gPsi || !gt || S <= 0 || nprobe <= 0 || (!V) == (!t) || every <= 0) return EM_ERR_INVALID;
@@ -785,55 +857,14 @@ em_status em_multislice_adjoint(em_plan p, const float* V, const void* t, int S,
   const float2* ck = reinterpret_cast<const float2*>(ckpt);
   const float2* tc = reinterpret_cast<const float2*>(t);
   float2* gtc = reinterpret_cast<float2*>(gt);
-  const int pb = row_pb(p, nprobe);
-  const int nparts = (nprobe + pb - 1) / pb;
-  EM_HIP(p, p->gpart.ensure(sizeof(float2) * N * nparts));
-  if (every > 1) {
-    // segment workspace: the recomputed waves entering each slice of a segment
-    EM_HIP(p, p->seg.ensure(sizeof(float2) * BN * every));
-    EM_HIP(p, p->segwave.ensure(sizeof(float2) * BN));
-  }
-  // lambda_{S-1} = FFT^H gPsi: column inverse now, row inverse inside the first backward row pass
-  EM_TRY(col(p, COL_PRE_IFFT | COL_STORE, col_args(p, lam, lam, nprobe)));
-  for (int k = S - 1; k >= 0; --k) {
-    const float2* ckk;
-    if (every == 1) {
-      ckk = ck + (long long)k * BN;
-    } else {
-      const int j = k / every, s0 = j * every;
-      if (k == S - 1 || k % every == every - 1) {
-        // recompute the waves entering slices s0..k from checkpoint j
-        float2* sw = p->segwave.as<float2>();
-        float2* sg = p->seg.as<float2>();
-        EM_HIP(p, hipMemcpyAsync(sw, ck + (long long)j * BN, sizeof(float2) * BN, hipMemcpyDeviceToDevice, p->stream));
-        for (int s = s0; s <= k; ++s) {
-          RowArgs r = row_args_skip(p, sw, sw, nprobe, s > s0, s < k);
-          r.ckpt = sg + (long long)(s - s0) * BN;
-          unsigned m = ROW_CKPT_STORE | (s > s0 ? ROW_PRE_IFFT : 0u);
-          if (s < k) {
-            m |= ROW_POST_FFT | (V ? ROW_MUL_V : ROW_MUL_T);
-            if (V) r.V = V + (long long)s * N; else r.t = tc + (long long)s * N;
-          }
-          EM_TRY(row(p, m, r));
-          if (s < k) EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, sw, sw, nprobe)));
-        }
-      }
-      ckk = p->seg.as<float2>() + (long long)(k - s0) * BN;
-    }
-    RowArgs r = row_args_skip(p, lam, lam, nprobe, k < S - 1, k > 0);
-    r.ckpt = const_cast<float2*>(ckk);
-    r.gpart = nparts == 1 ? gtc + (long long)k * N : p->gpart.as<float2>();
-    unsigned m = ROW_PRE_IFFT | ROW_GRAD;
-    if (k > 0 || gpsi0) {
-      m |= ROW_CONJ | (V ? ROW_MUL_V : ROW_MUL_T);
-      if (V) r.V = V + (long long)k * N; else r.t = tc + (long long)k * N;
-    }
-    if (k > 0) m |= ROW_POST_FFT;
-    else if (gpsi0) r.out = reinterpret_cast<float2*>(gpsi0);  // conj(t_0) lambda_0, real space
-    EM_TRY(row(p, m, r));
-    if (nparts > 1)
-      EM_HIP(p, launch_grad_reduce(N, nparts, p->gpart.as<float2>(), gtc + (long long)k * N, p->stream));
-    if (k > 0) EM_TRY(col(p, COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, lam, lam, nprobe)));
+  float2* g0 = reinterpret_cast<float2*>(gpsi0);
+  // probe groups as in the forward pass: a group's lambdas and checkpoints
+  // stay near the last-level cache for the whole backward sweep
+  const int G = group_size(p, nprobe);
+  for (int b0 = 0; b0 < nprobe; b0 += G) {
+    const int nb = std::min(G, nprobe - b0);
+    EM_TRY(adjoint_group(p, V, tc, S, ck + (long long)b0 * N, BN, every, lam + (long long)b0 * N, nb, gtc,
+                         g0 ? g0 + (long long)b0 * N : nullptr, b0 == 0));
   }
   if (V && gV) EM_HIP(p, launch_grad_v(gtc, V, 1.f, N * S, gV, p->stream));
   return EM_OK;
@@ -1393,8 +1424,8 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   if (st == EM_OK) st = measure(&cur);
   // coordinate descent over the launch-shape parameters, two sweeps
   const int groups[] = {0, 8, 16, 24, 32, 40, 48, 64, nprobe};
-  const int rowb[] = {1024, 2048, 4096, 8192, 16384};
-  const int colb[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int rowb[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int colb[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
   const int row_variants[] = {0, 1};
@@ -1406,7 +1437,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
                         : par == 3 ? tiles
                         : par == 4 ? variants
                                    : row_variants;
-      const int nc = par == 0 ? 9 : par == 1 ? 5 : par == 2 ? 6 : 2;
+      const int nc = par == 0 ? 9 : par == 1 ? 6 : par == 2 ? 7 : 2;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;

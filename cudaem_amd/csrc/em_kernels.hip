@@ -124,19 +124,34 @@ void row_pass_kernel(RowArgs a) {
     for (int e = 0; e < E; ++e) g[e] = make_float2(0.f, 0.f);
   }
 
+  // Short lines (8 values per lane): the next probe's row is loaded while
+  // this probe is transformed (as in the column pass).
+  constexpr bool PREFETCH = (E <= 8) && (MODE & ROW_IN_PLANE) == 0;
+  float2 nv[PREFETCH ? E : 1];
+  auto load_row = [&](long long rbase, float2* dst) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = tau + T * e;
+      const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
+      dst[e] = skip ? make_float2(0.f, 0.f) : a.in[rbase + kx];
+    }
+  };
+  if constexpr (PREFETCH) {
+    if (b0 < b1) load_row((long long)b0 * a.pstride + row, nv);
+  }
+
   for (int b = b0; b < b1; ++b) {
     const long long base = (long long)b * a.pstride + row;
     float2 v[E];
     if constexpr ((MODE & ROW_IN_PLANE) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = make_float2(1.f, 0.f);
-    } else {
+    } else if constexpr (PREFETCH) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int kx = tau + T * e;
-        const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
-        v[e] = skip ? make_float2(0.f, 0.f) : a.in[base + kx];
-      }
+      for (int e = 0; e < E; ++e) v[e] = nv[e];
+      if (b + 1 < b1) load_row(base + a.pstride, nv);
+    } else {
+      load_row(base, v);
     }
     if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
@@ -269,6 +284,18 @@ void col_pass_kernel(ColArgs a) {
     }
   }
 
+  // Short lines (8 values per lane) leave registers for the next probe's
+  // column: its loads are issued before this probe's transforms, so the
+  // workgroup does not wait for memory at the top of every probe.
+  constexpr bool PREFETCH = (E <= 8) && (MODE & (COL_LOAD_PROBE | COL_LOAD_MUL_A)) == 0;
+  float2 nv[PREFETCH ? E : 1];
+  if constexpr (PREFETCH) {
+    if (b0 < b1) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) nv[e] = a.in[(long long)b0 * a.pstride + x + (long long)(tau + T * e) * nx];
+    }
+  }
+
   for (int b = b0; b < b1; ++b) {
     const long long base = (long long)b * a.pstride + x;
     float2 v[E];
@@ -278,6 +305,14 @@ void col_pass_kernel(ColArgs a) {
       for (int e = 0; e < E; ++e) {
         const float2 eyv = a.ey[(long long)b * NY + tau + T * e];
         v[e] = cmul(cmul(p[e], exv), eyv);
+      }
+    } else if constexpr (PREFETCH) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = nv[e];
+      if (b + 1 < b1) {
+        const long long nbase = base + a.pstride;
+#pragma unroll
+        for (int e = 0; e < E; ++e) nv[e] = a.in[nbase + (long long)(tau + T * e) * nx];
       }
     } else {
 #pragma unroll
@@ -1018,17 +1053,18 @@ hipError_t launch_det_reduce(int nprobe, int ndet, int ntiles, const float* part
   return hipGetLastError();
 }
 
-__global__ void grad_reduce_kernel(long long n, int nparts, const float2* part, float2* g) {
+__global__ void grad_reduce_kernel(long long n, int nparts, const float2* part, float2* g, int accumulate) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    float2 s = make_float2(0.f, 0.f);
+    float2 s = accumulate ? g[i] : make_float2(0.f, 0.f);
     for (int p = 0; p < nparts; ++p) s = cadd(s, part[(long long)p * n + i]);
     g[i] = s;
   }
 }
 
-hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g, hipStream_t s) {
-  hipLaunchKernelGGL(grad_reduce_kernel, dim3(2048), dim3(256), 0, s, n, nparts, part, g);
+hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g, int accumulate,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(2048), dim3(256), 0, s, n, nparts, part, g, accumulate);
   return hipGetLastError();
 }
 

@@ -154,7 +154,8 @@ hipError_t launch_probe_ramps(const GridDev& g, int nprobe, const double* xe, co
 // out[d * ostride + b] (+)= w * sum_tiles part[b][d][tile]
 hipError_t launch_det_reduce(int nprobe, int ndet, int ntiles, const float* part, float w,
                              int accumulate, float* out, long long ostride, hipStream_t s);
-hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g,
+// g (+)= sum_p part[p]  (accumulate: add to g instead of overwriting it)
+hipError_t launch_grad_reduce(long long n, int nparts, const float2* part, float2* g, int accumulate,
                               hipStream_t s);
 hipError_t launch_fill(float2* p, long long n, float2 v, hipStream_t s);
 hipError_t launch_phase_fill(float2* p, long long n, hipStream_t s);

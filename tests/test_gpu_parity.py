@@ -421,6 +421,32 @@ def test_adjoint_recompute_and_potential_gradient(em):
     assert rel_l2(res[1][1], gV_o) < TOL_INT
 
 
+def test_adjoint_probe_groups_agree(em):
+    """The adjoint runs probe group by group and adds each group's gradient
+    into dL/dt in order: groups of 2 (with recompute) give the single-group
+    gradient and the same lambda_0."""
+    import torch
+    rng = np.random.default_rng(8)
+    n, S, B, L = 128, 5, 5, 14.0
+    V = torch.from_numpy(rng.uniform(0, 0.8, (S, n, n)).astype(np.float32)).cuda()
+    with em.Plan(n, n, L, L, 1.5, 300.0, max_batch=B) as p:
+        psi0 = p.build_probes(probe_lens(25.0), rng.uniform(0, L, B), rng.uniform(0, L, B))
+        out = {}
+        for grp, every in ((0, 1), (2, 1), (2, 2), (1, 3)):
+            p.set_probe_group(grp)
+            psi = psi0.clone()
+            ck = torch.empty((p.n_checkpoints(S, every), B, n, n), dtype=torch.complex64, device="cuda")
+            p.multislice_forward(psi, V=V, ckpt=ck, every=every)
+            g = (psi * 1e-6).contiguous()
+            gp0 = torch.empty_like(psi0)
+            gt = p.multislice_adjoint(ck, g, V=V, every=every, gpsi0=gp0)
+            out[(grp, every)] = (_np(gt), _np(gp0))
+    ref = out[(0, 1)]
+    for key, (gt, gp0) in out.items():
+        assert rel_l2(gt, ref[0]) < 1e-6, key
+        assert rel_l2(gp0, ref[1]) < 1e-6, key
+
+
 def test_autograd_function_matches_oracle_gradient(em):
     """Inverse-multislice loss of the differentiable-inversion workload shape
     (reduced): sum of squared far-field intensity differences; gradient w.r.t.
