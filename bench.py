@@ -145,7 +145,7 @@ def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
     return bw
 
 
-def adjoint_ratio(em, torch, n=512, S=40, scan=16, reps=3, autotune=True):
+def adjoint_ratio(em, torch, n=512, S=40, scan=16, reps=3):
     """Differentiable inverse multislice workload: 512^2, 40 slices of smooth
     random transmission (phase in [0, 0.5] rad, amplitude 0.95-1), 16 x 16
     probes at 300 kV / 20 mrad, loss = sum of squared far-field intensity
@@ -163,8 +163,6 @@ def adjoint_ratio(em, torch, n=512, S=40, scan=16, reps=3, autotune=True):
 
     t = (smooth(0.95, 1.0) * torch.exp(1j * smooth(0.0, 0.5))).to(torch.complex64).contiguous()
     plan = em.Plan(n, n, L, L, 2.0, 300.0, max_batch=scan * scan)
-    if autotune:
-        plan.autotune(scan * scan, nslices=8)
     xs, ys = em.specimen.raster(scan, scan, L, L)
     psi0 = plan.build_probes({"outer_aper_mrad": 20.0}, xs, ys)
     B = psi0.shape[0]
@@ -332,7 +330,7 @@ def main():
     copy_bw = copy_bandwidth(torch)
 
     plain_ps = plain_path(em, torch, em.specimen.vibrate(at0, U_RMS, SEED, 1), xs, ys, args.plain_probes)
-    t_fwd, t_fwd_adj = adjoint_ratio(em, torch, autotune=not args.no_autotune)
+    t_fwd, t_fwd_adj = adjoint_ratio(em, torch)
     rag_ps, rag_work = ragged_mix(em, torch)
 
     cpu = None
@@ -384,7 +382,7 @@ def main():
                                    "200/300 kV, 10-30 mrad, detector sums every 10 slices, one graph launch"},
         "adjoint_vs_forward": {"ratio": t_fwd_adj / t_fwd, "forward_s": t_fwd, "forward_plus_adjoint_s": t_fwd_adj,
                                "workload": "512^2, 40 slices, 16x16 probes, 300 kV / 20 mrad, "
-                                           "checkpoint every slice, autotuned plan"},
+                                           "checkpoint every slice, default launch shapes"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
