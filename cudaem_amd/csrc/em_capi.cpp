@@ -443,7 +443,7 @@ em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
 
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
   if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0 || t->col_variant < 0 ||
-      t->col_variant > 2 || t->row_variant < 0 || t->row_variant > 1)
+      t->col_variant > 2 || t->row_variant < 0 || t->row_variant > 2)
     return EM_ERR_INVALID;
   if (t->col_tile != 0 && t->col_tile != col_tile(plan->grid.ny) &&
       (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0) &&
@@ -1040,8 +1040,16 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
     for (int s = 0; s < S; ++s) {
       RowArgs r = row_args_skip(p, wave, wave, ng, true, s < S - 1);
-      r.t = p->Tstore.as<float2>() + (long long)s * N;
-      EM_TRY(timed(p, 0, [&] { return row(p, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), r); }));
+      unsigned rm;
+      if (p->tune.row_variant == 2) {
+        // the potential slice (4 B per pixel) with sincos once per workgroup
+        r.V = p->Vstore.as<float>() + (long long)s * N;
+        rm = ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT;
+      } else {
+        r.t = p->Tstore.as<float2>() + (long long)s * N;
+        rm = row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT);
+      }
+      EM_TRY(timed(p, 0, [&] { return row(p, rm, r); }));
       ColArgs cc = col_args_skip(p, wave, wave, ng);
       if (s < S - 1) {
         EM_TRY(timed(p, 1, [&] { return col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc); }));
@@ -1291,7 +1299,11 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
   const ColArgs c = col_args_skip(p, w, w, nprobe);
   em_status st = EM_OK;
   // warm up both kernels once
-  const unsigned rm = row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT);
+  unsigned rm = row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT);
+  if (p->tune.row_variant == 2) {
+    r.V = V;
+    rm = ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT;
+  }
   st = row(p, rm, r);
   if (st == EM_OK) st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, c);
   if (st == EM_OK) st = p->hip(hipEventRecord(e[0], p->stream), "event");
@@ -1391,6 +1403,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   if (!p || nprobe <= 0 || nslices <= 0) return EM_ERR_INVALID;
   const long long N = p->N();
   // timed on precomputed transmissions, the form the STEM slice loop runs
+  // (on potential slices for row_variant 2)
   float2* T = nullptr;
   float2* w = nullptr;
   EM_HIP(p, hipMalloc(&T, sizeof(float2) * N * nslices));
@@ -1398,6 +1411,8 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
     (void)hipFree(T);
     return p->fail(EM_ERR_ALLOC, "autotune scratch");
   }
+  // potential slices: the real parts of the unit phasors (any values do)
+  float* Vt = reinterpret_cast<float*>(T);
   hipEvent_t e0, e1;
   em_status st = p->hip(launch_phase_fill(T, N * nslices, p->stream), "phase fill");
   if (st == EM_OK) st = p->hip(launch_phase_fill(w, N * nprobe, p->stream), "phase fill");
@@ -1406,11 +1421,14 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   // best of three timed runs after a warm-up run: the minimum is the most
   // repeatable statistic when other work on the device adds noise
   auto measure = [&](double* ms) -> em_status {
-    EM_TRY(em_multislice_forward(p, nullptr, T, nslices, w, nprobe, nullptr, 1));  // warm
+    const bool vmode = p->tune.row_variant == 2;
+    const float* Vm = vmode ? Vt : nullptr;
+    const void* Tm = vmode ? nullptr : T;
+    EM_TRY(em_multislice_forward(p, Vm, Tm, nslices, w, nprobe, nullptr, 1));  // warm
     float best = 0;
     for (int i = 0; i < 3; ++i) {
       EM_HIP(p, hipEventRecord(e0, p->stream));
-      EM_TRY(em_multislice_forward(p, nullptr, T, nslices, w, nprobe, nullptr, 1));
+      EM_TRY(em_multislice_forward(p, Vm, Tm, nslices, w, nprobe, nullptr, 1));
       EM_HIP(p, hipEventRecord(e1, p->stream));
       EM_HIP(p, hipEventSynchronize(e1));
       float t = 0;
@@ -1428,7 +1446,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   const int colb[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
-  const int row_variants[] = {0, 1};
+  const int row_variants[] = {0, 1, 2};
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
     for (int par = 0; par < 6 && st == EM_OK; ++par) {
       const int* cand = par == 0   ? groups
@@ -1437,7 +1455,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
                         : par == 3 ? tiles
                         : par == 4 ? variants
                                    : row_variants;
-      const int nc = par == 0 ? 9 : par == 1 ? 6 : par == 2 ? 7 : 2;
+      const int nc = par == 0 ? 9 : par == 1 ? 6 : par == 2 ? 7 : par == 5 ? 3 : 2;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;

@@ -271,6 +271,61 @@ struct TwChain<N, T, NS, R, RS...> {
   }
 };
 
+// Twiddles of one stage read from a per-workgroup LDS table instead of being
+// formed in registers: entry (r - 1) * NS + k holds W_N^{r k step}, where k is
+// the butterfly index mod NS.  Lane tau's k is tau mod NS (T a multiple of
+// NS) or tau + T (U mod NS/T) (NS a multiple of T), so every twiddle is one
+// LDS read at a compile-time offset from the lane's base and one complex
+// multiply; lanes read consecutive (or broadcast) entries.  Stages whose k
+// follows neither pattern cannot use the table (ok == false).
+template <int N, int T, int NS, int R>
+struct StageTwTab {
+  static constexpr int step = N / (NS * R);
+  static constexpr int mode = (NS <= 1) ? 3 : ((T % NS == 0) ? 1 : ((NS % T == 0) ? 2 : 0));
+  static constexpr int entries = (NS <= 1) ? 0 : (R - 1) * NS;
+  const float2* base;
+
+  __device__ __forceinline__ void init(int tau, const float2* tab) {
+    base = tab + ((mode == 1) ? (tau % NS) : tau);
+  }
+  template <int DIR, int U, int RR>
+  __device__ __forceinline__ float2 apply(float2 v, const float2* __restrict__) const {
+    constexpr int off = (RR - 1) * NS + ((mode == 2) ? T * (U % (NS / T)) : 0);
+    return twmul<DIR>(v, base[off]);
+  }
+  __device__ __forceinline__ static void fill(float2* tab, const float2* __restrict__ tw, int tid, int nt) {
+    for (int i = tid; i < entries; i += nt) {
+      const int r = i / NS + 1, k = i % NS;
+      tab[i] = tw[(r * k * step) % N];
+    }
+  }
+};
+
+template <int N, int T, int NS, int... RS>
+struct TwChainTab {
+  static constexpr int entries = 0;
+  static constexpr bool ok = true;
+  __device__ __forceinline__ void init(int, const float2*) {}
+  __device__ __forceinline__ static void fill(float2*, const float2*, int, int) {}
+};
+template <int N, int T, int NS, int R, int... RS>
+struct TwChainTab<N, T, NS, R, RS...> {
+  using St = StageTwTab<N, T, NS, R>;
+  using Next = TwChainTab<N, T, NS * R, RS...>;
+  static constexpr int entries = St::entries + Next::entries;
+  static constexpr bool ok = (St::mode != 0) && Next::ok;
+  St s;
+  Next next;
+  __device__ __forceinline__ void init(int tau, const float2* tab) {
+    s.init(tau, tab);
+    next.init(tau, tab + St::entries);
+  }
+  __device__ __forceinline__ static void fill(float2* tab, const float2* __restrict__ tw, int tid, int nt) {
+    St::fill(tab, tw, tid, nt);
+    Next::fill(tab + St::entries, tw, tid, nt);
+  }
+};
+
 // Synchronisation between the LDS write and read phases of an exchange.
 // SYNC_BLOCK: lines span several wavefronts (workgroup barrier).
 // SYNC_WAVE:  every line lives inside one wavefront; LDS requests of one wave
@@ -304,8 +359,8 @@ struct FftLine {
     }
   }
 
-  template <int R, int NS, int U, int RR>
-  __device__ __forceinline__ static void twiddle_one(float2 (&v)[E], const StageTw<N, T, NS, R>& st,
+  template <int R, int NS, int U, int RR, class St>
+  __device__ __forceinline__ static void twiddle_one(float2 (&v)[E], const St& st,
                                                      const float2* __restrict__ tw) {
     if constexpr (U < E / R) {
       if constexpr (RR < R) {
@@ -334,9 +389,9 @@ struct FftLine {
     for (int e = 0; e < E; ++e) v[e] = lds[LineLds<N>::idx(tau + T * e)];
   }
 
-  template <int NS, int R, int... RS>
-  __device__ __forceinline__ static void stages(float2 (&v)[E], float2* lds, int tau,
-                                                const TwChain<N, T, NS, R, RS...>& tws,
+  // Chain: TwChain (twiddles formed in registers) or TwChainTab (LDS table)
+  template <int NS, int R, int... RS, class Chain>
+  __device__ __forceinline__ static void stages(float2 (&v)[E], float2* lds, int tau, const Chain& tws,
                                                 const float2* __restrict__ tw) {
     static_assert(E % R == 0, "radix must divide the elements per thread");
     if constexpr (NS > 1) twiddle_one<R, NS, 0, 1>(v, tws.s, tw);
@@ -355,6 +410,12 @@ struct FftLine {
   template <int... RS>
   __device__ __forceinline__ static void run(float2 (&v)[E], float2* lds, int tau,
                                              const TwChain<N, T, 1, RS...>& tws,
+                                             const float2* __restrict__ tw) {
+    stages<1, RS...>(v, lds, tau, tws, tw);
+  }
+  template <int... RS>
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2* lds, int tau,
+                                             const TwChainTab<N, T, 1, RS...>& tws,
                                              const float2* __restrict__ tw) {
     stages<1, RS...>(v, lds, tau, tws, tw);
   }

@@ -8,6 +8,8 @@
 // the wave once.  Between passes the wave is kept in (y, kx) layout, row-major.
 // Both passes reuse the transmission / propagator values held in registers
 // for every probe a workgroup processes.
+#include <type_traits>
+
 #include "em_fft.hpp"
 #include "em_kernels.hpp"
 
@@ -23,9 +25,10 @@ struct FftCfg;
   struct FftCfg<N_> {                                                               \
     static constexpr int T = T_;                                                    \
     using Tw = TwChain<N_, T_, 1, __VA_ARGS__>;                                     \
-    template <int DIR, int SYNC>                                                    \
+    using TwTab = TwChainTab<N_, T_, 1, __VA_ARGS__>;                               \
+    template <int DIR, int SYNC, class Chain>                                       \
     __device__ __forceinline__ static void run(float2 (&v)[N_ / T_], float2* lds,   \
-                                               int tau, const Tw& tws,              \
+                                               int tau, const Chain& tws,           \
                                                const float2* tw) {                  \
       FftLine<N_, T_, DIR, SYNC>::template run<__VA_ARGS__>(v, lds, tau, tws, tw);  \
     }                                                                               \
@@ -99,8 +102,18 @@ void row_pass_kernel(RowArgs a) {
   float2* lds = smem + line * LineLds<NX>::size;
   const long long row = (long long)y * NX;
   constexpr int SYNC = (T <= 64) ? SYNC_WAVE : SYNC_BLOCK;  // a line inside one wavefront
-  typename FftCfg<NX>::Tw tws;
-  tws.init(tau, a.tw);
+  // Twiddles: a per-workgroup LDS table after the line buffers where every
+  // stage allows it (one read per twiddle instead of forming it in registers)
+  constexpr bool TWTAB = FftCfg<NX>::TwTab::ok;
+  std::conditional_t<TWTAB, typename FftCfg<NX>::TwTab, typename FftCfg<NX>::Tw> tws;
+  if constexpr (TWTAB) {
+    float2* tab = smem + LINES * LineLds<NX>::size;
+    FftCfg<NX>::TwTab::fill(tab, a.tw, threadIdx.x, kRowThreads);
+    __syncthreads();
+    tws.init(tau, tab);
+  } else {
+    tws.init(tau, a.tw);
+  }
 
   // transmission values of this thread's elements, reused for every probe
   float2 t[(MODE & ROW_T_STREAM) ? 1 : E];
@@ -494,7 +507,8 @@ template <int NX>
 static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   constexpr int LINES = kRowThreads / FftCfg<NX>::T;
   const dim3 grid(a.ny / LINES, (a.nprobe + a.pb - 1) / a.pb);
-  const size_t lds = (size_t)LINES * LineLds<NX>::size * sizeof(float2);
+  constexpr int TAB = FftCfg<NX>::TwTab::ok ? FftCfg<NX>::TwTab::entries : 0;
+  const size_t lds = ((size_t)LINES * LineLds<NX>::size + TAB) * sizeof(float2);
   switch (mode) {
 #define EM_ROW_CASE(M)                                                            \
   case (M):                                                                       \

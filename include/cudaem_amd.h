@@ -106,8 +106,9 @@ em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* 
  * last-level cache can hold); col_variant: exchange scheme of the per-slice
  * column pass (0 = automatic, 1 = workgroup barrier per exchange,
  * 2 = one wavefront per column after the first stage); row_variant: row
- * pass of the STEM slice loop (0 = transmission held in registers, 1 = read
- * per probe, four waves per SIMD). */
+ * pass of the STEM slice loop (0 = precomputed transmission held in
+ * registers, 1 = read per probe, four waves per SIMD, 2 = potential slice
+ * read and its sincos held in registers: half the table traffic). */
 typedef struct em_tuning {
   int row_blocks, col_blocks, col_tile, probe_group, col_variant, row_variant;
 } em_tuning;

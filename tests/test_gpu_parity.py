@@ -267,20 +267,23 @@ def test_stem_probes_exit_waves_and_detectors(em):
 
 
 def test_stem_batches_and_groups_agree(em):
-    """em_stem with several batches / probe groups equals one big batch."""
+    """em_stem with several batches / probe groups / row-pass variants equals
+    one big batch."""
     import cudaem_amd.specimen as sp
     at = sp.srtio3(3, 3, 4)
     n = 128
     xs, ys = sp.raster(5, 5, at["lx"], at["ly"])
     lens = probe_lens(20.0)
     res = []
-    for mb, grp in ((25, 0), (8, 3), (1, 1)):
+    for mb, grp, rv in ((25, 0, 0), (8, 3, 0), (1, 1, 0), (8, 3, 1), (8, 3, 2)):
         with em.Plan(n, n, at["lx"], at["ly"], 1.9525, 300.0, max_batch=mb) as p:
             p.set_probe_group(grp)
+            p.set_tuning(row_variant=rv)
             p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
             res.append(_np(p.stem(lens, xs, ys, [40.0], [120.0])))
-    assert rel_l2(res[1], res[0]) < 1e-6
-    assert rel_l2(res[2], res[0]) < 1e-6
+    # row variants: transmission read per probe, or the potential with sincos per workgroup
+    for r in res[1:]:
+        assert rel_l2(r, res[0]) < 1e-6
 
 
 def test_stem_pass_timing_runs_the_same_passes(em):
