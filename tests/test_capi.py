@@ -153,3 +153,58 @@ def test_host_spline_matches_oracle(C, Z):
     ref = O.build_spline(Z)
     for got, exp in zip(arrs, ref):
         np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+
+
+# ----------------------------------------------------------------- property-based host logic
+from hypothesis import given, settings, strategies as st  # noqa: E402
+
+
+@settings(max_examples=60, deadline=None)
+@given(z=st.lists(st.floats(-50.0, 50.0, allow_nan=False), min_size=1, max_size=300),
+       dz=st.floats(0.3, 5.0), border=st.integers(0, 2), r_int=st.floats(0.0, 4.0), fine=st.booleans())
+def test_host_slicing_property_matches_oracle(C, z, dz, border, r_int, fine):
+    """Random z sets, slice thicknesses, borders and integration radii: the
+    product's slicer equals the oracle's bit for bit, and every atom of a
+    direct slicing sits in exactly one slice."""
+    z = np.ascontiguousarray(np.sort(np.asarray(z, dtype=np.float64)))
+    cap = 4096
+    ns = ctypes.c_int()
+    b = np.zeros(cap + 1)
+    i0 = np.zeros(cap, np.int64)
+    i1 = np.zeros(cap, np.int64)
+    args = (len(z), z.ctypes.data_as(C._PD), dz, border)
+    if fine:
+        st_ = C.lib().em_host_slicing_fine(*args, r_int, cap, ctypes.byref(ns), b.ctypes.data_as(C._PD),
+                                           i0.ctypes.data_as(C._PLL), i1.ctypes.data_as(C._PLL))
+        bo, i0o, i1o = O.slicing_fine(z, dz, border, r_int)
+    else:
+        st_ = C.lib().em_host_slicing(*args, cap, ctypes.byref(ns), b.ctypes.data_as(C._PD),
+                                      i0.ctypes.data_as(C._PLL), i1.ctypes.data_as(C._PLL))
+        bo, i0o, i1o = O.slicing(z, dz, border)
+    assert st_ == 0
+    S = ns.value
+    assert np.array_equal(b[:S + 1], bo) and np.array_equal(i0[:S], i0o) and np.array_equal(i1[:S], i1o)
+    if not fine:
+        counts = np.zeros(len(z), int)
+        for s in range(S):
+            if i0[s] <= i1[s]:
+                counts[i0[s]:i1[s] + 1] += 1
+        assert (counts <= 1).all()
+
+
+@settings(max_examples=40, deadline=None)
+@given(n=st.integers(1, 40), coefs=st.lists(st.floats(-3.0, 3.0), min_size=1, max_size=80))
+def test_host_quadratures_are_exact_on_polynomials(C, n, coefs):
+    """Gauss-Legendre on [-1/2, 1/2] and Gauss-Hermite (weight e^{-t^2})
+    integrate polynomials of degree < 2n exactly."""
+    c = np.asarray(coefs[:2 * n])
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert C.lib().em_host_quadrature(0, n, x.ctypes.data_as(C._PD), w.ctypes.data_as(C._PD)) == 0
+    exact = sum(ck * (0.0 if k % 2 else 2 * 0.5 ** (k + 1) / (k + 1)) for k, ck in enumerate(c))
+    assert abs(np.dot(w, np.polyval(c[::-1], x)) - exact) < 1e-12 * max(1.0, np.abs(c).sum())
+    assert C.lib().em_host_quadrature(1, n, x.ctypes.data_as(C._PD), w.ctypes.data_as(C._PD)) == 0
+    from math import gamma
+    exact = sum(ck * (0.0 if k % 2 else gamma((k + 1) / 2)) for k, ck in enumerate(c))
+    scale = max(1.0, sum(abs(ck) * gamma((k + 1) / 2) for k, ck in enumerate(c)))
+    assert abs(np.dot(w, np.polyval(c[::-1], x)) - exact) < 1e-10 * scale
