@@ -37,7 +37,7 @@ HBM_PEAK = 8.0e12
 # 40 probes, band skip 1e-9, row_blocks 4096 / col_blocks 8192: rocprofv3 --pmc
 # FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000), factors from a 1 GiB device
 # copy; profiles/batched_stem_1024_hbm_traffic.md
-PMC_BYTES_PER_PROBE = {"column pass": (280.6 + 247.7) / 40 * 2**20, "row pass": (305.2 + 241.0) / 40 * 2**20}
+PMC_BYTES_PER_PROBE = {"column pass": (281.4 + 247.9) / 40 * 2**20, "row pass": (301.1 + 241.0) / 40 * 2**20}
 
 
 def parse():

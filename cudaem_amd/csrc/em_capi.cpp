@@ -254,23 +254,29 @@ em_status col(em_plan p, unsigned mode, const ColArgs& a) {
   return p->hip(launch_col(mode, a, p->stream), "column pass");
 }
 
-// Launch through `run`, bracketed by a pair of timing events when the plan
-// records per-launch pass times (kind 0 = row pass, 1 = column pass).
+// Launch through `run` (one row or column pass) with a pair of timing events
+// that the launch itself records, when the plan records per-launch pass times
+// (kind 0 = row pass, 1 = column pass).
 template <typename F>
 em_status timed(em_plan p, int kind, F&& run) {
   if (!p->pass_timing) return run();
+  // timing-only events: no system-scope cache writeback between the passes,
+  // as between the passes of a graph launch
   hipEvent_t e[2];
-  EM_HIP(p, hipEventCreate(&e[0]));
-  if (hipEventCreate(&e[1]) != hipSuccess) {
+  EM_HIP(p, hipEventCreateWithFlags(&e[0], hipEventDisableSystemFence));
+  if (hipEventCreateWithFlags(&e[1], hipEventDisableSystemFence) != hipSuccess) {
     (void)hipEventDestroy(e[0]);
     return p->fail(EM_ERR_HIP, "event create");
   }
   p->tev.push_back(e[0]);
   p->tev.push_back(e[1]);
   p->tkind.push_back(kind);
-  EM_HIP(p, hipEventRecord(e[0], p->stream));
-  EM_TRY(run());
-  EM_HIP(p, hipEventRecord(e[1], p->stream));
+  set_pass_events(e[0], e[1]);
+  const em_status st = run();
+  const bool taken = !pass_events_pending();
+  set_pass_events(nullptr, nullptr);
+  EM_TRY(st);
+  if (!taken) return p->fail(EM_ERR_HIP, "timed launch started no pass");
   return EM_OK;
 }
 

@@ -8,12 +8,39 @@
 // the wave once.  Between passes the wave is kept in (y, kx) layout, row-major.
 // Both passes reuse the transmission / propagator values held in registers
 // for every probe a workgroup processes.
+#include <hip/hip_ext.h>
+
+#include <cstdint>
 #include <type_traits>
 
 #include "em_fft.hpp"
 #include "em_kernels.hpp"
 
 namespace em {
+
+// Start / stop events armed for the next row or column pass launched from
+// this thread (set_pass_events).  A pass launch takes them, so they carry the
+// kernel's own begin and end timestamps.
+static thread_local hipEvent_t t_pass_start = nullptr, t_pass_stop = nullptr;
+
+void set_pass_events(hipEvent_t start, hipEvent_t stop) {
+  t_pass_start = start;
+  t_pass_stop = stop;
+}
+
+bool pass_events_pending() { return t_pass_start != nullptr; }
+
+template <typename K, typename... A>
+static hipError_t launch_pass(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, const A&... args) {
+  if (t_pass_start != nullptr) {
+    hipEvent_t e0 = t_pass_start, e1 = t_pass_stop;
+    t_pass_start = t_pass_stop = nullptr;
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, s, e0, e1, 0u, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+  }
+  return hipGetLastError();
+}
 
 // --------------------------------------------------------------------------
 // FFT configurations per line length: threads per line and radix sequence.
@@ -512,8 +539,7 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   switch (mode) {
 #define EM_ROW_CASE(M)                                                            \
   case (M):                                                                       \
-    hipLaunchKernelGGL((row_pass_kernel<NX, (M)>), grid, dim3(kRowThreads), lds, s, a); \
-    return hipGetLastError();
+    return launch_pass(row_pass_kernel<NX, (M)>, grid, dim3(kRowThreads), lds, s, a);
     EM_ROW_MODES(EM_ROW_CASE)
 #undef EM_ROW_CASE
     default:
@@ -540,8 +566,7 @@ static hipError_t col_launch_one(const ColArgs& args, hipStream_t s) {
   ColArgs a = args;
   const int skipped = skipped_tiles<COLS>(M, a);
   const dim3 grid(a.nx / C::COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
-  hipLaunchKernelGGL((col_pass_kernel<NY, COLS, M>), grid, dim3(C::THREADS), C::LDS, s, a);
-  return hipGetLastError();
+  return launch_pass(col_pass_kernel<NY, COLS, M>, grid, dim3(C::THREADS), C::LDS, s, a);
 }
 
 template <int NY>
@@ -608,8 +633,7 @@ static hipError_t col_launch_t(const ColArgs& args, hipStream_t s) {
     ColArgs a = args;
     const int skipped = skipped_tiles<COLS>(M, a);
     const dim3 grid(a.nx / COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
-    hipLaunchKernelGGL((col_pass_t_kernel<NY, COLS, M>), grid, dim3(COLS * 64), lds, s, a);
-    return hipGetLastError();
+    return launch_pass(col_pass_t_kernel<NY, COLS, M>, grid, dim3(COLS * 64), lds, s, a);
   } else {
     return hipErrorInvalidValue;
   }

@@ -112,6 +112,12 @@ enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };
 
 hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
 hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
+// Arms a pair of timing events for the next row or column pass launched from
+// the calling thread: that launch records the kernel's own start and end in
+// them (hipExtLaunchKernel), without the dispatch gap that events queued
+// around a launch also measure.  pass_events_pending: armed and not yet taken.
+void set_pass_events(hipEvent_t start, hipEvent_t stop);
+bool pass_events_pending();
 // Raises the dynamic-LDS limit of every column-pass kernel of length ny
 // (call once per device before launching them).
 hipError_t prepare_col_kernels(int ny);
