@@ -44,13 +44,6 @@ struct DevBuf {
   }
 };
 
-struct GraphKey {
-  int kind, B, S;
-  bool operator<(const GraphKey& o) const {
-    return kind != o.kind ? kind < o.kind : (B != o.B ? B < o.B : S < o.S);
-  }
-};
-
 struct Graph {
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
@@ -104,7 +97,6 @@ struct em_plan_s {
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
   int vstore_slices = -1;
-  std::map<GraphKey, std::unique_ptr<Graph>> graphs;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
   // per-launch timing of the STEM slice loop's row / column passes: event
   // pairs (start, end) of each timed launch, kind 0 = row, 1 = column
@@ -118,7 +110,6 @@ struct em_plan_s {
   }
   ~em_plan_s() {
     clear_pass_events();
-    graphs.clear();
     if (gstream) (void)hipStreamDestroy(gstream);
   }
 
@@ -372,7 +363,6 @@ em_status em_plan_set_band_skip(em_plan plan, double threshold, int* skip_lo, in
   if (!plan || threshold < 0 || threshold > 1e-6) return EM_ERR_INVALID;
   plan->band_skip = threshold;
   compute_band_skip(plan);
-  plan->graphs.clear();
   if (skip_lo) *skip_lo = plan->skip_lo;
   if (skip_hi) *skip_hi = plan->skip_hi;
   return EM_OK;
@@ -388,7 +378,6 @@ em_status em_plan_destroy(em_plan plan) {
 em_status em_plan_set_stream(em_plan plan, void* s) {
   if (!plan) return EM_ERR_INVALID;
   plan->stream = reinterpret_cast<hipStream_t>(s);
-  plan->graphs.clear();
   return EM_OK;
 }
 
@@ -403,7 +392,6 @@ em_status em_plan_constants(em_plan plan, double* lambda, double* sigma, double*
 em_status em_plan_set_probe_group(em_plan plan, int g) {
   if (!plan || g < 0) return EM_ERR_INVALID;
   plan->tune.probe_group = g;
-  plan->graphs.clear();
   return EM_OK;
 }
 
@@ -456,7 +444,6 @@ em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
       !(t->col_tile == 8 && col_transposed_available(plan->grid.ny) && plan->grid.nx % 8 == 0))
     return EM_ERR_INVALID;
   plan->tune = *t;
-  plan->graphs.clear();
   return EM_OK;
 }
 
@@ -646,7 +633,6 @@ em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n
   p->fine_entries = p->fine ? (long long)az_h.size() : 0;
   p->has_specimen = true;
   p->vstore_slices = -1;
-  p->graphs.clear();
   if (n_slices) *n_slices = S;
   return EM_OK;
 }
@@ -1488,7 +1474,6 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
       p->tune = bestt;
     }
   }
-  p->graphs.clear();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(p->stream);
