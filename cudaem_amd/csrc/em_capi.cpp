@@ -96,6 +96,7 @@ struct em_plan_s {
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
+  DevBuf chunks;  // chunk tables of the ragged path
   int vstore_slices = -1;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
   // per-launch timing of the STEM slice loop's row / column passes: event
@@ -1191,18 +1192,134 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   if (nreq <= 0 || !plans || !V || !S || !psi || !nprobe || !out || ndet <= 0 || ndet > kMaxDet || !inner ||
       !outer || stride <= 0)
     return EM_ERR_INVALID;
-  for (int r = 0; r < nreq; ++r)
+  for (int r = 0; r < nreq; ++r) {
     if (!plans[r] || !V[r] || !psi[r] || S[r] <= 0 || nprobe[r] <= 0 || !out[r]) return EM_ERR_INVALID;
+    for (int q = 0; q < r; ++q)
+      if (plans[q] == plans[r]) return plans[0]->fail(EM_ERR_INVALID, "requests must use distinct plans");
+  }
   hipStream_t user = reinterpret_cast<hipStream_t>(hip_stream);
   em_plan p0 = plans[0];
-  // size every buffer before capture
+
+  // Classes: requests of one grid size and band-skip range step through their
+  // slices together, and every fused pass of a slice step is one launch over
+  // all of their probes (a chunk table gives each workgroup row its request's
+  // wave and V or P).  Detector passes stay per request (their bands depend
+  // on the request's wavelength).
+  std::vector<std::vector<int>> cls;
+  for (int r = 0; r < nreq; ++r) {
+    em_plan p = plans[r];
+    size_t c = 0;
+    for (; c < cls.size(); ++c) {
+      em_plan q = plans[cls[c][0]];
+      if (q->grid.nx == p->grid.nx && q->grid.ny == p->grid.ny && q->skip_lo == p->skip_lo &&
+          q->skip_hi == p->skip_hi)
+        break;
+    }
+    if (c == cls.size()) cls.emplace_back();
+    cls[c].push_back(r);
+  }
+  // The launches of every class in order, with their chunk tables.
+  enum OpKind { OP_ROW, OP_DET, OP_COL };
+  struct Op {
+    OpKind kind;
+    int first, count;  // chunk range (OP_ROW, OP_COL)
+    unsigned mode;     // OP_ROW
+    int skip_in, skip_out;
+    int req, io;       // OP_DET
+  };
+  std::vector<PassChunk> chunks;
+  std::vector<std::vector<Op>> ops(cls.size());
+  for (size_t c = 0; c < cls.size(); ++c) {
+    em_plan pc = plans[cls[c][0]];
+    const long long N = pc->N();
+    int Smax = 0;
+    for (int r : cls[c]) Smax = std::max(Smax, S[r]);
+    std::vector<int> io(nreq, 0);
+    auto add_chunks = [&](const std::vector<int>& reqs, int pb, bool row_table, int s) {
+      const int first = (int)chunks.size();
+      for (int r : reqs)
+        for (int j = 0; j < nprobe[r]; j += pb) {
+          PassChunk ch{};
+          ch.out = reinterpret_cast<float2*>(psi[r]) + (long long)j * N;
+          ch.in = ch.out;
+          ch.table = row_table ? static_cast<const void*>(V[r] + (long long)s * N)
+                               : static_cast<const void*>(plans[r]->P.as<float2>());
+          ch.n = std::min(pb, nprobe[r] - j);
+          chunks.push_back(ch);
+        }
+      return std::make_pair(first, (int)chunks.size() - first);
+    };
+    for (int s = 0; s < Smax; ++s) {
+      const bool rec_step = (s + 1) % stride == 0;
+      // row pass: one launch per output-skip setting (a request's last slice
+      // keeps every column for its detector pass)
+      for (int so = 0; so < 2; ++so) {
+        std::vector<int> reqs;
+        int btot = 0;
+        for (int r : cls[c]) {
+          if (s >= S[r]) continue;
+          const bool last = s == S[r] - 1;
+          if ((!rec_step && !last ? 1 : 0) != so) continue;
+          reqs.push_back(r);
+          btot += nprobe[r];
+        }
+        if (reqs.empty()) continue;
+        const auto fc = add_chunks(reqs, row_pb(pc, btot), true, s);
+        Op op{};
+        op.kind = OP_ROW;
+        op.first = fc.first;
+        op.count = fc.second;
+        op.mode = (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS;
+        op.skip_in = s > 0;
+        op.skip_out = so;
+        ops[c].push_back(op);
+      }
+      // detector sums of the recorded slices
+      for (int r : cls[c]) {
+        if (s >= S[r] || !(rec_step || s == S[r] - 1)) continue;
+        Op op{};
+        op.kind = OP_DET;
+        op.req = r;
+        op.io = io[r]++;
+        ops[c].push_back(op);
+      }
+      // propagator column pass of every request with slices left
+      std::vector<int> reqs;
+      int btot = 0;
+      for (int r : cls[c])
+        if (s < S[r] - 1) {
+          reqs.push_back(r);
+          btot += nprobe[r];
+        }
+      if (!reqs.empty()) {
+        const int blocks = pc->grid.nx / col_tile(pc->grid.ny);
+        const int target = std::max(1, pc->tune.col_blocks);
+        const int groups = std::max(1, std::min(btot, (target + blocks - 1) / blocks));
+        const auto fc = add_chunks(reqs, (btot + groups - 1) / groups, false, s);
+        Op op{};
+        op.kind = OP_COL;
+        op.first = fc.first;
+        op.count = fc.second;
+        ops[c].push_back(op);
+      }
+    }
+  }
+  for (const auto& oc : ops)
+    for (const Op& op : oc)
+      if (op.kind != OP_DET && op.count > 65535) return p0->fail(EM_ERR_INVALID, "too many probes in one pass");
+
+  // size and fill every buffer before capture
   for (int r = 0; r < nreq; ++r) {
     em_plan p = plans[r];
     const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
     EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)nprobe[r] * ndet * ntiles));
   }
+  EM_HIP(p0, p0->chunks.ensure(sizeof(PassChunk) * chunks.size()));
+  EM_HIP(p0, hipMemcpy(p0->chunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
+  const PassChunk* dchunks = p0->chunks.as<PassChunk>();
+
   hipEvent_t fork, join;
-  std::vector<hipEvent_t> done(nreq);
+  std::vector<hipEvent_t> done(cls.size());
   EM_HIP(p0, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
   EM_HIP(p0, hipEventCreateWithFlags(&join, hipEventDisableTiming));
   for (auto& e : done) EM_HIP(p0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1210,56 +1327,54 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   EM_HIP(p0, hipStreamWaitEvent(p0->gstream, fork, 0));
   Graph graph;
   em_status st = p0->hip(hipStreamBeginCapture(p0->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
-  hipEvent_t cap_fork;
+  hipEvent_t cap_fork = nullptr;
   if (st == EM_OK) st = p0->hip(hipEventCreateWithFlags(&cap_fork, hipEventDisableTiming), "event");
   if (st == EM_OK) st = p0->hip(hipEventRecord(cap_fork, p0->gstream), "record");
-  for (int r = 0; r < nreq && st == EM_OK; ++r) {
-    em_plan p = plans[r];
-    // each request is a branch of the graph on its plan's private stream
-    hipStream_t br = (r == 0) ? p0->gstream : p->gstream;
-    if (r > 0 && br == p0->gstream) {
-      st = p0->fail(EM_ERR_INVALID, "requests must use distinct plans");
-      break;
-    }
-    if (r > 0) st = p->hip(hipStreamWaitEvent(br, cap_fork, 0), "fork");
-    if (st != EM_OK) break;
-    StreamSwap sw(p, br);
-    const long long N = p->N();
-    const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
-    double dg2min[kMaxDet], dg2max[kMaxDet];
-    for (int d = 0; d < ndet; ++d) {
-      dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
-      dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
-    }
-    float2* wv = reinterpret_cast<float2*>(psi[r]);
-    const int B = nprobe[r];
-    int io = 0;
-    for (int s = 0; s < S[r] && st == EM_OK; ++s) {
-      const bool rec = ((s + 1) % stride == 0) || s == S[r] - 1;
-      RowArgs ra = row_args_skip(p, wv, wv, B, s > 0, !rec && s < S[r] - 1);
-      ra.V = V[r] + (long long)s * N;
-      st = row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, ra);
-      if (st != EM_OK) break;
-      if (rec) {
-        ColArgs cd = col_args(p, wv, nullptr, B);
+  for (size_t c = 0; c < cls.size() && st == EM_OK; ++c) {
+    // each class is a branch of the graph on its first plan's private stream
+    // (class 0 holds request 0)
+    em_plan pc = plans[cls[c][0]];
+    hipStream_t br = pc->gstream;
+    if (c > 0) st = pc->hip(hipStreamWaitEvent(br, cap_fork, 0), "fork");
+    for (size_t k = 0; k < ops[c].size() && st == EM_OK; ++k) {
+      const Op& op = ops[c][k];
+      if (op.kind == OP_ROW) {
+        RowArgs ra = row_args(pc, nullptr, nullptr, op.count);
+        ra.pb = 1;
+        ra.chunks = dchunks + op.first;
+        ra.skip_in = op.skip_in;
+        ra.skip_out = op.skip_out;
+        st = pc->hip(launch_row(op.mode, ra, br), "row pass");
+      } else if (op.kind == OP_COL) {
+        ColArgs cc = col_args(pc, nullptr, nullptr, op.count);
+        cc.pb = 1;
+        cc.chunks = dchunks + op.first;
+        cc.cols = 0;
+        cc.variant = COL_VARIANT_BLOCK;
+        cc.skip_lo = pc->skip_lo;
+        cc.skip_hi = pc->skip_hi;
+        st = pc->hip(launch_col(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS, cc, br),
+                     "column pass");
+      } else {
+        const int r = op.req;
+        em_plan p = plans[r];
+        const int B = nprobe[r];
+        ColArgs cd = col_args(p, reinterpret_cast<float2*>(psi[r]), nullptr, B);
         cd.ndet = ndet;
         for (int d = 0; d < ndet; ++d) {
-          cd.dg2min[d] = dg2min[d];
-          cd.dg2max[d] = dg2max[d];
+          cd.dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+          cd.dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
         }
         cd.dpart = p->dpart.as<float>();
-        st = col(p, COL_PRE_FFT | COL_DET, cd);
+        st = p->hip(launch_col(COL_PRE_FFT | COL_DET, cd, br), "column pass");
         if (st == EM_OK)
-          st = p->hip(launch_det_reduce(B, ndet, ntiles, p->dpart.as<float>(), 1.f, 0,
-                                        out[r] + (long long)io * ndet * B, B, p->stream),
+          st = p->hip(launch_det_reduce(B, ndet, col_ntiles(p->grid.nx, p->grid.ny), p->dpart.as<float>(), 1.f, 0,
+                                        out[r] + (long long)op.io * ndet * B, B, br),
                       "detector reduce");
-        ++io;
       }
-      if (st == EM_OK && s < S[r] - 1)
-        st = col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args_skip(p, wv, wv, B));
     }
-    if (st == EM_OK) st = p->hip(hipEventRecord(done[r], br), "record");
-    if (st == EM_OK && r > 0) st = p->hip(hipStreamWaitEvent(p0->gstream, done[r], 0), "join");
+    if (st == EM_OK) st = pc->hip(hipEventRecord(done[c], br), "record");
+    if (st == EM_OK && c > 0) st = pc->hip(hipStreamWaitEvent(p0->gstream, done[c], 0), "join");
   }
   const hipError_t ec = hipStreamEndCapture(p0->gstream, &graph.g);
   if (st == EM_OK) st = p0->hip(ec, "end capture");
@@ -1270,7 +1385,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   if (st == EM_OK) st = p0->hip(hipStreamSynchronize(p0->gstream), "sync");
   (void)hipEventDestroy(fork);
   (void)hipEventDestroy(join);
-  (void)hipEventDestroy(cap_fork);
+  if (cap_fork) (void)hipEventDestroy(cap_fork);
   for (auto& e : done) (void)hipEventDestroy(e);
   return st;
 }

@@ -42,6 +42,37 @@ static hipError_t launch_pass(K kernel, dim3 grid, dim3 block, size_t lds, hipSt
   return hipGetLastError();
 }
 
+// A chunk-table entry read by a whole workgroup (its index is blockIdx.y).
+// Its pointers are device (global) memory, stated so through an address-space
+// cast: pointers loaded from memory are otherwise generic, and generic
+// accesses need a 64-bit per-lane address each (spills in the column pass).
+// readfirstlane keeps the uniform fields in scalar registers.
+using GlobalF2 = __attribute__((address_space(1))) float2;
+using GlobalF = __attribute__((address_space(1))) float;
+struct UniformChunk {
+  const float2* in;
+  float2* out;
+  const float2* P;
+  const float* V;
+  int n;
+};
+__device__ __forceinline__ UniformChunk read_chunk(const PassChunk* c) {
+  const PassChunk ch = *c;
+  auto u64 = [](const void* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+  };
+  UniformChunk u;
+  u.in = (const float2*)reinterpret_cast<const GlobalF2*>(u64(ch.in));
+  u.out = (float2*)reinterpret_cast<GlobalF2*>(u64(ch.out));
+  u.P = (const float2*)reinterpret_cast<const GlobalF2*>(u64(ch.table));
+  u.V = (const float*)reinterpret_cast<const GlobalF*>(u64(ch.table));
+  u.n = __builtin_amdgcn_readfirstlane(ch.n);
+  return u;
+}
+
 // --------------------------------------------------------------------------
 // FFT configurations per line length: threads per line and radix sequence.
 // --------------------------------------------------------------------------
@@ -124,8 +155,19 @@ void row_pass_kernel(RowArgs a) {
   const int line = threadIdx.x / T;
   const int tau = threadIdx.x % T;
   const int y = blockIdx.x * LINES + line;
-  const int b0 = blockIdx.y * a.pb;
-  const int b1 = min(a.nprobe, b0 + a.pb);
+  const float2* in = a.in;
+  float2* out = a.out;
+  const float* Vsl = a.V;
+  int b0 = blockIdx.y * a.pb;
+  int b1 = min(a.nprobe, b0 + a.pb);
+  if constexpr ((MODE & ROW_CHUNKS) != 0) {
+    const UniformChunk ch = read_chunk(a.chunks + blockIdx.y);
+    in = ch.in;
+    out = ch.out;
+    Vsl = ch.V;
+    b0 = 0;
+    b1 = ch.n;
+  }
   float2* lds = smem + line * LineLds<NX>::size;
   const long long row = (long long)y * NX;
   constexpr int SYNC = (T <= 64) ? SYNC_WAVE : SYNC_BLOCK;  // a line inside one wavefront
@@ -150,7 +192,7 @@ void row_pass_kernel(RowArgs a) {
       const long long i = row + tau + T * e;
       if constexpr ((MODE & ROW_MUL_V) != 0) {
         float s, c;
-        sincosf(a.V[i], &s, &c);
+        sincosf(Vsl[i], &s, &c);
         t[e] = make_float2(c, s);
       } else {
         t[e] = a.t[i];
@@ -173,7 +215,7 @@ void row_pass_kernel(RowArgs a) {
     for (int e = 0; e < E; ++e) {
       const int kx = tau + T * e;
       const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
-      dst[e] = skip ? make_float2(0.f, 0.f) : a.in[rbase + kx];
+      dst[e] = skip ? make_float2(0.f, 0.f) : in[rbase + kx];
     }
   };
   if constexpr (PREFETCH) {
@@ -238,7 +280,7 @@ void row_pass_kernel(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int kx = tau + T * e;
-        if (!(a.skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) a.out[base + kx] = v[e];
+        if (!(a.skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) out[base + kx] = v[e];
       }
     }
   }
@@ -288,8 +330,19 @@ void col_pass_kernel(ColArgs a) {
     if (a.tile_lo <= a.tile_hi && tile >= a.tile_lo) tile += a.tile_hi - a.tile_lo + 1;
   const int x = tile * COLS + c;
   const int nx = a.nx;
-  const int b0 = blockIdx.y * a.pb;
-  const int b1 = min(a.nprobe, b0 + a.pb);
+  const float2* in = a.in;
+  float2* out = a.out;
+  const float2* Ptab = a.P;
+  int b0 = blockIdx.y * a.pb;
+  int b1 = min(a.nprobe, b0 + a.pb);
+  if constexpr ((MODE & COL_CHUNKS) != 0) {
+    const UniformChunk ch = read_chunk(a.chunks + blockIdx.y);
+    in = ch.in;
+    out = ch.out;
+    Ptab = ch.P;
+    b0 = 0;
+    b1 = ch.n;
+  }
   float2* lds = smem + c * C::STRIDE;
   typename FftCfg<NY>::Tw tws;
   tws.init(tau, a.tw);
@@ -298,7 +351,7 @@ void col_pass_kernel(ColArgs a) {
   if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      p[e] = a.P[(long long)(tau + T * e) * nx + x];
+      p[e] = Ptab[(long long)(tau + T * e) * nx + x];
       if constexpr ((MODE & COL_CONJ_P) != 0) p[e].y = -p[e].y;
     }
   } else if constexpr ((MODE & (COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0) {
@@ -332,7 +385,7 @@ void col_pass_kernel(ColArgs a) {
   if constexpr (PREFETCH) {
     if (b0 < b1) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) nv[e] = a.in[(long long)b0 * a.pstride + x + (long long)(tau + T * e) * nx];
+      for (int e = 0; e < E; ++e) nv[e] = in[(long long)b0 * a.pstride + x + (long long)(tau + T * e) * nx];
     }
   }
 
@@ -352,11 +405,11 @@ void col_pass_kernel(ColArgs a) {
       if (b + 1 < b1) {
         const long long nbase = base + a.pstride;
 #pragma unroll
-        for (int e = 0; e < E; ++e) nv[e] = a.in[nbase + (long long)(tau + T * e) * nx];
+        for (int e = 0; e < E; ++e) nv[e] = in[nbase + (long long)(tau + T * e) * nx];
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = a.in[base + (long long)(tau + T * e) * nx];
+      for (int e = 0; e < E; ++e) v[e] = in[base + (long long)(tau + T * e) * nx];
       if constexpr ((MODE & COL_LOAD_MUL_A) != 0) {
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
@@ -397,7 +450,7 @@ void col_pass_kernel(ColArgs a) {
     if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & COL_STORE) != 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) a.out[base + (long long)(tau + T * e) * nx] = v[e];
+      for (int e = 0; e < E; ++e) out[base + (long long)(tau + T * e) * nx] = v[e];
     }
   }
 }
@@ -478,6 +531,8 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   X(ROW_PRE_IFFT)                                                               \
   X(ROW_MUL_V | ROW_POST_FFT)                                                   \
   X(ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT)                                    \
+  X(ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS)                                      \
+  X(ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS)                       \
   X(ROW_IN_PLANE | ROW_MUL_V | ROW_POST_FFT)                                    \
   X(ROW_MUL_T | ROW_POST_FFT)                                                   \
   X(ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT)                                    \
@@ -503,6 +558,7 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   X(COL_PRE_FFT | COL_STORE)                                                    \
   X(COL_PRE_IFFT | COL_STORE)                                                   \
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
+  X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS)           \
   X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)                       \
   X(COL_PRE_FFT | COL_DET)                                                      \
   X(COL_PRE_FFT | COL_DET | COL_STORE)                                          \

@@ -30,6 +30,19 @@ struct LensDev {
   double env_lambda = 0, env_c10 = 0, env_c30 = 0, env_sigma = 0, env_beta = 0;
 };
 
+// One entry of a chunk table (ragged batches): n probes of one request,
+// pstride elements apart, read from `in` and written to `out`, with that
+// request's own table -- the potential slice V (row pass) or the propagator
+// P (column pass).  The passes run in place (in == out); the two fields keep
+// the kernels from holding every load address for the matching store.
+struct PassChunk {
+  const float2* in;
+  float2* out;
+  const void* table;
+  int n;
+  int pad;
+};
+
 // --- row pass ---------------------------------------------------------------
 enum RowMode : unsigned {
   ROW_PRE_IFFT = 1u << 0,    // inverse FFT along x before the multiply
@@ -45,6 +58,8 @@ enum RowMode : unsigned {
   ROW_OUT_ACCUM = 1u << 10,  // accumulate weight * output into the FP32 planes
   ROW_T_STREAM = 1u << 11,   // read t for every probe instead of holding it in
                              // registers (frees 2E VGPRs for a fourth wave per SIMD)
+  ROW_CHUNKS = 1u << 12,     // in place; workgroup row y takes its probes, wave and
+                             // V from chunks[y] (launch with nprobe = chunks, pb = 1)
 };
 
 struct RowArgs {
@@ -65,6 +80,7 @@ struct RowArgs {
   // skip_in: those input columns are taken as zero and not read;
   // skip_out: those output columns are not written.
   int skip_lo, skip_hi, skip_in, skip_out;
+  const PassChunk* chunks;  // ROW_CHUNKS: one entry per workgroup row
 };
 
 // --- column pass ------------------------------------------------------------
@@ -80,6 +96,9 @@ enum ColMode : unsigned {
   COL_LOAD_MUL_A = 1u << 8,   // value = wave * A
   COL_STORE = 1u << 9,
   COL_DET = 1u << 10,         // detector partial sums over closed bands
+  COL_CHUNKS = 1u << 11,      // in place; workgroup row y takes its probes, wave and
+                              // P from chunks[y] (launch with nprobe = chunks, pb = 1;
+                              // default tile, block variant)
 };
 
 constexpr int kMaxDet = 8;
@@ -106,6 +125,7 @@ struct ColArgs {
   // neither read nor written (set from skip_lo/hi by the launcher)
   int skip_lo, skip_hi;
   int tile_lo, tile_hi;
+  const PassChunk* chunks;  // COL_CHUNKS: one entry per workgroup row
 };
 
 enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };

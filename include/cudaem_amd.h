@@ -308,11 +308,15 @@ em_status em_thickness(em_plan plan, const float* V_dev, int S, void* psi_dev, i
                        double inner_mrad, double outer_mrad, int stride, float w, int accumulate,
                        float* frames_dev, int* n_frames);
 
-/* Ragged batch: requests on different grids run as one HIP graph launch; all
- * requests of one grid size share each fused pass.  Request r: plan plans[r]
- * (its specimen set), psi_dev[r] incident waves [B_r][N_r] (overwritten),
- * S[r] slices of V_dev[r], detector sums every `stride` slices into
- * out_dev[r][nout_r][ndet][B_r] where nout_r = ceil(S_r/stride). */
+/* Ragged batch: requests on different grids run as one HIP graph launch.
+ * Requests of one grid size (and band-limit skip range) step through their
+ * slices together: each row and propagator column pass of a slice step is one
+ * launch over all of their probes, every probe with its own request's
+ * potential and propagator; detector passes run per request.  Request r:
+ * plan plans[r] (distinct plans), psi_dev[r] incident waves [B_r][N_r]
+ * (overwritten), S[r] slices of V_dev[r], detector sums every `stride`
+ * slices and after the last one into out_dev[r][nout_r][ndet][B_r] where
+ * nout_r = ceil(S_r/stride). */
 em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V_dev, const int* S,
                                void* const* psi_dev, const int* nprobe, int ndet,
                                const double* inner_mrad, const double* outer_mrad, int stride,
