@@ -259,10 +259,14 @@ struct StageTw {
 
 template <int N, int T, int NS, int... RS>
 struct TwChain {
+  static constexpr bool in_registers = true;
   __device__ __forceinline__ void init(int, const float2*) {}
 };
 template <int N, int T, int NS, int R, int... RS>
 struct TwChain<N, T, NS, R, RS...> {
+  // false when some stage reads the global table at every use (mode 0)
+  static constexpr bool in_registers =
+      StageTw<N, T, NS, R>::mode != 0 && TwChain<N, T, NS * R, RS...>::in_registers;
   StageTw<N, T, NS, R> s;
   TwChain<N, T, NS * R, RS...> next;
   __device__ __forceinline__ void init(int tau, const float2* __restrict__ tw) {
@@ -277,21 +281,34 @@ struct TwChain<N, T, NS, R, RS...> {
 // NS) or tau + T (U mod NS/T) (NS a multiple of T), so every twiddle is one
 // LDS read at a compile-time offset from the lane's base and one complex
 // multiply; lanes read consecutive (or broadcast) entries.  Stages whose k
-// follows neither pattern cannot use the table (ok == false).
+// follows neither pattern (mode 0, e.g. NS = 12 or 48 with T = 64) keep the
+// lane's k of each of its E/R butterflies, computed once at init.
 template <int N, int T, int NS, int R>
 struct StageTwTab {
   static constexpr int step = N / (NS * R);
   static constexpr int mode = (NS <= 1) ? 3 : ((T % NS == 0) ? 1 : ((NS % T == 0) ? 2 : 0));
   static constexpr int entries = (NS <= 1) ? 0 : (R - 1) * NS;
+  static constexpr int NB = N / T / R;  // butterflies per lane
   const float2* base;
+  int kb[mode == 0 ? NB : 1];
 
   __device__ __forceinline__ void init(int tau, const float2* tab) {
-    base = tab + ((mode == 1) ? (tau % NS) : tau);
+    if constexpr (mode == 0) {
+      base = tab;
+#pragma unroll
+      for (int u = 0; u < NB; ++u) kb[u] = (tau + T * u) % NS;
+    } else {
+      base = tab + ((mode == 1) ? (tau % NS) : tau);
+    }
   }
   template <int DIR, int U, int RR>
   __device__ __forceinline__ float2 apply(float2 v, const float2* __restrict__) const {
-    constexpr int off = (RR - 1) * NS + ((mode == 2) ? T * (U % (NS / T)) : 0);
-    return twmul<DIR>(v, base[off]);
+    if constexpr (mode == 0) {
+      return twmul<DIR>(v, base[(RR - 1) * NS + kb[U]]);
+    } else {
+      constexpr int off = (RR - 1) * NS + ((mode == 2) ? T * (U % (NS / T)) : 0);
+      return twmul<DIR>(v, base[off]);
+    }
   }
   __device__ __forceinline__ static void fill(float2* tab, const float2* __restrict__ tw, int tid, int nt) {
     for (int i = tid; i < entries; i += nt) {
@@ -313,7 +330,7 @@ struct TwChainTab<N, T, NS, R, RS...> {
   using St = StageTwTab<N, T, NS, R>;
   using Next = TwChainTab<N, T, NS * R, RS...>;
   static constexpr int entries = St::entries + Next::entries;
-  static constexpr bool ok = (St::mode != 0) && Next::ok;
+  static constexpr bool ok = true;
   St s;
   Next next;
   __device__ __forceinline__ void init(int tau, const float2* tab) {

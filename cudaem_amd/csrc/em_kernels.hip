@@ -132,7 +132,12 @@ struct ColCfg {
   static constexpr int COLS = COLS_;
   static constexpr int THREADS = COLS * T;
   static constexpr int STRIDE = LineLds<N>::size + 1;  // odd stride: columns on distinct banks
-  static constexpr size_t LDS = (size_t)COLS * STRIDE * sizeof(float2);
+  // Twiddles from an LDS table after the column buffers where the register
+  // form would read the global table at every use (768); elsewhere they are
+  // formed in registers.
+  static constexpr bool TWTAB = !FftCfg<N>::Tw::in_registers;
+  static constexpr int TAB = TWTAB ? FftCfg<N>::TwTab::entries : 0;
+  static constexpr size_t LDS = ((size_t)COLS * STRIDE + TAB) * sizeof(float2);
 };
 
 // --------------------------------------------------------------------------
@@ -304,6 +309,17 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
 
 // At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
 // 8-column tile then share a CU, one loading while the other computes.
+// Whether the column pass reads P right before the multiply instead of
+// holding it in registers through the forward FFT.  Held, it makes the 768
+// and 2048 kernels spill at 4 waves per SIMD (16 B and 148 B of scratch);
+// read late they spill 0 B and 20 B, and the 2048 pass runs 24% faster per
+// probe (26.0 against 34.0 us, full width).  At 512 and 1024 the held P is as
+// fast or faster (1024: 4.0 against 5.05 us per probe).
+template <int NY>
+struct ColLateP {
+  static constexpr bool value = NY == 768 || NY == 2048;
+};
+
 template <int NY, int COLS_, unsigned MODE>
 __global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 void col_pass_kernel(ColArgs a) {
@@ -344,11 +360,19 @@ void col_pass_kernel(ColArgs a) {
     b1 = ch.n;
   }
   float2* lds = smem + c * C::STRIDE;
-  typename FftCfg<NY>::Tw tws;
-  tws.init(tau, a.tw);
+  std::conditional_t<C::TWTAB, typename FftCfg<NY>::TwTab, typename FftCfg<NY>::Tw> tws;
+  if constexpr (C::TWTAB) {
+    float2* tab = smem + COLS * C::STRIDE;
+    FftCfg<NY>::TwTab::fill(tab, a.tw, threadIdx.x, C::THREADS);
+    __syncthreads();
+    tws.init(tau, tab);
+  } else {
+    tws.init(tau, a.tw);
+  }
 
-  float2 p[(MODE & (COL_MUL_P | COL_CONJ_P | COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0 ? E : 1];
-  if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
+  constexpr bool LATE_P = ColLateP<NY>::value && (MODE & (COL_MUL_P | COL_CONJ_P)) != 0;
+  float2 p[(MODE & (COL_MUL_P | COL_CONJ_P | COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0 && !LATE_P ? E : 1];
+  if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0 && !LATE_P) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       p[e] = Ptab[(long long)(tau + T * e) * nx + x];
@@ -417,7 +441,14 @@ void col_pass_kernel(ColArgs a) {
     }
     if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
-    if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
+    if constexpr (LATE_P) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        float2 pe = Ptab[(long long)(tau + T * e) * nx + x];
+        if constexpr ((MODE & COL_CONJ_P) != 0) pe.y = -pe.y;
+        v[e] = cmul(v[e], pe);
+      }
+    } else if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
     }
