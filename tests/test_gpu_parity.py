@@ -266,6 +266,46 @@ def test_stem_probes_exit_waves_and_detectors(em):
     assert rel_l2(stem, ref_det) < TOL_INT
 
 
+@pytest.mark.parametrize("n", [768, 2048])
+def test_probe_multislice_large_grids(em, n):
+    """Probe exit waves and detector sums at 768^2 (radix-12 stages with LDS
+    twiddle tables) and 2048^2 (two wavefronts per column line); both read
+    the propagator late in the column pass.  A thin SrTiO3 block keeps the
+    oracle's FP64 FFTs short."""
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 2)
+    dz, e0 = 1.9525, 300.0
+    lens = probe_lens(25.0, c10=20.0)
+    xs = np.array([2.1, 6.9])
+    ys = np.array([5.5, 1.7])
+    inner, outer = np.array([40.0, 10.0]), np.array([160.0, 40.0])
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=2) as p:
+        S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        zs, ze, _, _ = p.slice_info(0)
+        V = p.potential_slices()
+        psi = p.build_probes(lens, xs, ys, defocus_shift=0.5 * (zs + ze))
+        p.multislice_forward(psi, V=V)
+        det = _np(p.detector_sums(psi, inner, outer))
+        Psi = _np(psi)
+        torch.cuda.synchronize()
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
+    assert S == o.S
+    Vo = o.potentials()
+    Po = o.propagator()
+    g2 = [(np.sin(a * 1e-3) / o.lam) ** 2 for a in (inner, outer)]
+    ref = np.zeros((len(inner), len(xs)))
+    for b in range(len(xs)):
+        psi0 = O.probe(n, n, at["lx"], at["ly"], o.lam, lens, xs[b], ys[b], o.zmid0())
+        Psi_o = O.multislice(Po, psi0, V=Vo)
+        e = rel_l2(Psi[b], Psi_o)
+        print(f"n={n} probe {b} exit-wave relL2 {e:.3e}")
+        assert e < TOL_WAVE
+        for d in range(len(inner)):
+            ref[d, b] = O.detector_sum(Psi_o, at["lx"], at["ly"], g2[0][d], g2[1][d])
+    assert rel_l2(det, ref) < TOL_INT
+
+
 def test_non_square_grid_exit_waves_and_stem(em):
     """A 256 x 128 grid over a 4 x 2 x 4 SrTiO3 block (square pixels): exit
     waves of probes, detector sums and the batched STEM path against the
