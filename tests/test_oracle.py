@@ -61,7 +61,7 @@ def test_constants_match_scipy_golden():
     for kv, lam, gam, sig in zip(GOLD["kv"], GOLD["lambda"], GOLD["gamma"], GOLD["sigma"]):
         l, g, s = O.constants(kv)
         assert abs(l / lam - 1) < 1e-9 and abs(g / gam - 1) < 1e-9 and abs(s / sig - 1) < 1e-9
-    # the survey's table (rounded published values)
+    # the closed forms' values at 200 and 300 kV, rounded to 5-6 significant digits
     l, g, s = O.constants(200.0)
     assert abs(l - 0.025079) < 1e-6 and abs(g - 1.391390) < 1e-6 and abs(s - 7.2884e-4) < 1e-8
     l, g, s = O.constants(300.0)
