@@ -33,7 +33,7 @@ EXPORTS = [
     "em_set_element_coefs", "em_set_spline_params", "em_set_element_params", "em_plan_set_fine_slicing",
     "em_plan_set_specimen", "em_slice_info",
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
-    "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_adjoint",
+    "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_forward_multi", "em_multislice_adjoint",
     "em_detector_sums", "em_ctf_image", "em_pctf_image", "em_tcc_image", "em_diffraction", "em_stem", "em_thickness",
     "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_quadrature", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
@@ -110,6 +110,7 @@ _SIGS = {
     "em_plane_wave": ([_P, _I, _P], _I),
     "em_fft2d": ([_P, _P, _I, _I], _I),
     "em_multislice_forward": ([_P, _P, _P, _I, _P, _I, _P, _I], _I),
+    "em_multislice_forward_multi": ([_P, _P, _P, _P, _I], _I),
     "em_multislice_adjoint": ([_P, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _I),
     "em_detector_sums": ([_P, _P, _I, _I, _PD, _PD, _F, _I, _P], _I),
     "em_ctf_image": ([_P, ctypes.POINTER(em_lens), _P, _I, _P, _P, _P, _F, _I], _I),

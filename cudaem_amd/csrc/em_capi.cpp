@@ -96,7 +96,8 @@ struct em_plan_s {
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
-  DevBuf chunks;  // chunk tables of the ragged path
+  DevBuf chunks;   // chunk tables of the ragged path
+  DevBuf mchunks;  // chunk tables of em_multislice_forward_multi
   int vstore_slices = -1;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
   // per-launch timing of the STEM slice loop's row / column passes: event
@@ -765,6 +766,103 @@ em_status em_multislice_forward(em_plan p, const float* V, const void* t, int S,
     float2* ck = ckpt ? reinterpret_cast<float2*>(ckpt) + (long long)g0 * p->N() : nullptr;
     EM_TRY(forward_group(p, V, reinterpret_cast<const float2*>(t), S, w + (long long)g0 * p->N(), nb, ck,
                          (long long)nprobe * p->N(), every, false));
+  }
+  return EM_OK;
+}
+
+em_status em_multislice_forward_multi(em_plan p, const float* const* V, const int* S, void* psi, int nprobe) {
+  if (!p || !V || !S || !psi || nprobe <= 0) return EM_ERR_INVALID;
+  for (int b = 0; b < nprobe; ++b)
+    if (!V[b] || S[b] <= 0) return EM_ERR_INVALID;
+  const long long N = p->N();
+  float2* w = reinterpret_cast<float2*>(psi);
+  const float2* P = p->P.as<float2>();
+  // Every launch's chunk table is built before any launch is enqueued.  A
+  // chunk is a run of consecutive probes that share one potential stack.
+  struct Launch {
+    bool row;
+    unsigned mode;
+    int skip_in, skip_out, first, count;
+  };
+  std::vector<PassChunk> chunks;
+  std::vector<Launch> launches;
+  auto add = [&](const std::vector<int>& probes, int pb, bool row_table, int s) {
+    const int first = (int)chunks.size();
+    for (size_t i = 0; i < probes.size();) {
+      const int b = probes[i];
+      int n = 1;
+      while (i + n < probes.size() && n < pb && probes[i + n] == b + n && V[b + n] == V[b]) ++n;
+      PassChunk ch{};
+      ch.out = w + (long long)b * N;
+      ch.in = ch.out;
+      ch.table = row_table ? static_cast<const void*>(V[b] + (long long)s * N) : static_cast<const void*>(P);
+      ch.n = n;
+      chunks.push_back(ch);
+      i += n;
+    }
+    return std::make_pair(first, (int)chunks.size() - first);
+  };
+  const int blocks = p->grid.nx / col_tile(p->grid.ny);
+  auto col_chunk_pb = [&](int btot) {
+    const int groups = std::max(1, std::min(btot, (std::max(1, p->tune.col_blocks) + blocks - 1) / blocks));
+    return (btot + groups - 1) / groups;
+  };
+  const int G = group_size(p, nprobe);
+  for (int g0 = 0; g0 < nprobe; g0 += G) {
+    const int g1 = std::min(nprobe, g0 + G);
+    int Smax = 0;
+    for (int b = g0; b < g1; ++b) Smax = std::max(Smax, S[b]);
+    for (int s = 0; s < Smax; ++s) {
+      // row pass: output columns below the band-skip threshold are dropped
+      // except on a probe's last slice (its final column transform keeps all)
+      for (int so = 0; so < 2; ++so) {
+        std::vector<int> probes;
+        for (int b = g0; b < g1; ++b)
+          if (s < S[b] && (s < S[b] - 1 ? 1 : 0) == so) probes.push_back(b);
+        if (probes.empty()) continue;
+        const auto fc = add(probes, row_pb(p, (int)probes.size()), true, s);
+        launches.push_back({true, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS, s > 0, so,
+                            fc.first, fc.second});
+      }
+      // column pass: the propagator step, or the last slice's FFT_y
+      for (int last = 0; last < 2; ++last) {
+        std::vector<int> probes;
+        for (int b = g0; b < g1; ++b)
+          if (last ? s == S[b] - 1 : s < S[b] - 1) probes.push_back(b);
+        if (probes.empty()) continue;
+        const auto fc = add(probes, col_chunk_pb((int)probes.size()), false, s);
+        launches.push_back({false,
+                            last ? (COL_PRE_FFT | COL_STORE | COL_CHUNKS)
+                                 : (COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS),
+                            0, 0, fc.first, fc.second});
+      }
+    }
+  }
+  for (const Launch& l : launches)
+    if (l.count > 65535) return p->fail(EM_ERR_INVALID, "too many probes in one pass");
+  // an earlier call's launches may still be reading the table
+  EM_HIP(p, hipStreamSynchronize(p->stream));
+  EM_HIP(p, p->mchunks.ensure(sizeof(PassChunk) * chunks.size()));
+  EM_HIP(p, hipMemcpy(p->mchunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
+  const PassChunk* dchunks = p->mchunks.as<PassChunk>();
+  for (const Launch& l : launches) {
+    if (l.row) {
+      RowArgs ra = row_args(p, nullptr, nullptr, l.count);
+      ra.pb = 1;
+      ra.chunks = dchunks + l.first;
+      ra.skip_in = l.skip_in;
+      ra.skip_out = l.skip_out;
+      EM_TRY(row(p, l.mode, ra));
+    } else {
+      ColArgs cc = col_args(p, nullptr, nullptr, l.count);
+      cc.pb = 1;
+      cc.chunks = dchunks + l.first;
+      cc.cols = 0;
+      cc.variant = COL_VARIANT_BLOCK;
+      cc.skip_lo = p->skip_lo;
+      cc.skip_hi = p->skip_hi;
+      EM_TRY(col(p, l.mode, cc));
+    }
   }
   return EM_OK;
 }

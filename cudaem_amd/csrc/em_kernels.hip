@@ -590,6 +590,7 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   X(COL_PRE_IFFT | COL_STORE)                                                   \
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS)           \
+  X(COL_PRE_FFT | COL_STORE | COL_CHUNKS)                                       \
   X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)                       \
   X(COL_PRE_FFT | COL_DET)                                                      \
   X(COL_PRE_FFT | COL_DET | COL_STORE)                                          \

@@ -252,43 +252,71 @@ void run_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T>* o)
   o->init(m.ny, m.nx, 0, full);
   o->interval_x = m.lx / m.nx;
   o->interval_y = m.ly / m.ny;
-  Dev psi(N * 8), a(N * 4), b(N * 4);
+  // Frozen-phonon configurations run as the probes of one batch, each with its
+  // own potential stack (em_multislice_forward_multi), in blocks of at most
+  // kMaxConfBlock configurations and about kConfBlockBytes of potentials; the
+  // reference runs them one after another (single_simulator.cuh:60-80).
+  constexpr int kMaxConfBlock = 64;
+  constexpr size_t kConfBlockBytes = size_t(8) << 30;
+  const int kmax = std::min(nconf, kMaxConfBlock);
+  Dev psi(N * 8 * (size_t)kmax), a(N * 4), b(N * 4);
   hip_ok(hipMemset(a.p, 0, N * 4), "memset");
   hip_ok(hipMemset(b.p, 0, N * 4), "memset");
   const em_lens obj = lens_of(m.obj_lens), cond = lens_of(m.cond_lens);
-  for (int iconf = 1; iconf <= nconf; ++iconf) {
-    const Atom_Coord_Data<T> at = atoms_of_config(m, iconf);
-    const int S = set_specimen(plan.p, at, border);
-    Dev V((size_t)S * N * 4);
-    ok(em_potential_slices(plan.p, 0, S, V.as<float>()), plan.p, "em_potential_slices");
-    if (m.iw_type == eIWT_Convergent_Wave) {
-      const double xb = m.incident_wave_x_pos, yb = m.incident_wave_y_pos;
-      ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, psi.p), plan.p, "em_build_probes");
-    } else {
-      ok(em_plane_wave(plan.p, 1, psi.p), plan.p, "em_plane_wave");
+  for (int iconf = 1; iconf <= nconf;) {
+    std::vector<std::unique_ptr<Dev>> V;
+    std::vector<const float*> Vp;
+    std::vector<int> Ss;
+    size_t used = 0;
+    while (iconf <= nconf && (int)V.size() < kmax && used < kConfBlockBytes) {
+      const Atom_Coord_Data<T> at = atoms_of_config(m, iconf);
+      const int S = set_specimen(plan.p, at, border);
+      V.push_back(std::make_unique<Dev>((size_t)S * N * 4));
+      ok(em_potential_slices(plan.p, 0, S, V.back()->as<float>()), plan.p, "em_potential_slices");
+      Vp.push_back(V.back()->as<float>());
+      Ss.push_back(S);
+      used += (size_t)S * N * 4;
+      void* slot = psi.as<float2>() + (long long)(V.size() - 1) * (long long)N;
+      if (m.iw_type == eIWT_Convergent_Wave) {
+        // the probe's defocus is referred to this configuration's first slice
+        const double xb = m.incident_wave_x_pos, yb = m.incident_wave_y_pos;
+        ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, slot), plan.p, "em_build_probes");
+      } else {
+        ok(em_plane_wave(plan.p, 1, slot), plan.p, "em_plane_wave");
+      }
+      ++iconf;
     }
-    ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr, 1), plan.p, "em_multislice_forward");
-    if (m.simulation_type == eTEMST_HRTEM) {
-      // Lens_Modulation::wave_modulation's illumination switch (lens_modulation.cuh:52-73)
-      float* I = full ? nullptr : a.as<float>();
-      float* re = full ? a.as<float>() : nullptr;
-      float* im = full ? b.as<float>() : nullptr;
-      em_coherence coh;
-      coh.dsf_sigma = m.obj_lens.dsf_sigma;
-      coh.dsf_npoints = m.obj_lens.dsf_npoints;
-      coh.ssf_beta = m.obj_lens.ssf_beta;
-      coh.ssf_npoints = m.obj_lens.ssf_npoints;
-      coh.mode = (int)m.temp_spat_incoh;
-      if (m.illumination_model == eIM_Partial_Coherent)
-        ok(em_pctf_image(plan.p, &obj, &coh, psi.p, 1, I, re, im, w, 1), plan.p, "em_pctf_image");
-      else if (m.illumination_model == eIM_Trans_Cross_Coef)
-        ok(em_tcc_image(plan.p, &obj, &coh, psi.p, 1, I, re, im, w, 1), plan.p, "em_tcc_image");
-      else
-        ok(em_ctf_image(plan.p, &obj, psi.p, 1, I, re, im, w, 1), plan.p, "em_ctf_image");
-    } else if (full) {
-      hip_ok(em::launch_split(psi.as<float2>(), a.as<float>(), b.as<float>(), (long long)N, w, 1, nullptr), "split");
-    } else {
-      hip_ok(em::launch_abs2(psi.as<float2>(), a.as<float>(), (long long)N, w, 1, nullptr), "abs2");
+    const int k = (int)V.size();
+    if (k == 1)
+      ok(em_multislice_forward(plan.p, Vp[0], nullptr, Ss[0], psi.p, 1, nullptr, 1), plan.p, "em_multislice_forward");
+    else
+      ok(em_multislice_forward_multi(plan.p, Vp.data(), Ss.data(), psi.p, k), plan.p, "em_multislice_forward_multi");
+    for (int j = 0; j < k; ++j) {
+      void* wj = psi.as<float2>() + (long long)j * (long long)N;
+      if (m.simulation_type == eTEMST_HRTEM) {
+        // Lens_Modulation::wave_modulation's illumination switch (lens_modulation.cuh:52-73)
+        float* I = full ? nullptr : a.as<float>();
+        float* re = full ? a.as<float>() : nullptr;
+        float* im = full ? b.as<float>() : nullptr;
+        em_coherence coh;
+        coh.dsf_sigma = m.obj_lens.dsf_sigma;
+        coh.dsf_npoints = m.obj_lens.dsf_npoints;
+        coh.ssf_beta = m.obj_lens.ssf_beta;
+        coh.ssf_npoints = m.obj_lens.ssf_npoints;
+        coh.mode = (int)m.temp_spat_incoh;
+        if (m.illumination_model == eIM_Partial_Coherent)
+          ok(em_pctf_image(plan.p, &obj, &coh, wj, 1, I, re, im, w, 1), plan.p, "em_pctf_image");
+        else if (m.illumination_model == eIM_Trans_Cross_Coef)
+          ok(em_tcc_image(plan.p, &obj, &coh, wj, 1, I, re, im, w, 1), plan.p, "em_tcc_image");
+        else
+          ok(em_ctf_image(plan.p, &obj, wj, 1, I, re, im, w, 1), plan.p, "em_ctf_image");
+      } else if (full) {
+        hip_ok(em::launch_split(static_cast<const float2*>(wj), a.as<float>(), b.as<float>(), (long long)N, w, 1,
+                                nullptr),
+               "split");
+      } else {
+        hip_ok(em::launch_abs2(static_cast<const float2*>(wj), a.as<float>(), (long long)N, w, 1, nullptr), "abs2");
+      }
     }
     hip_ok(hipDeviceSynchronize(), "sync");
   }

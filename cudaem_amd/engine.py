@@ -274,6 +274,23 @@ class Plan:
         self._call("em_multislice_forward", _ptr(V), _ptr(t), S, _ptr(psi), B, _ptr(ckpt), int(every))
         return psi
 
+    def multislice_forward_multi(self, psi, Vs):
+        """psi: incident waves [B][ny][nx] (real space) -> exit waves (Fourier,
+        unnormalised), in place; probe b runs through its own potential stack
+        Vs[b] ([S_b][ny][nx] float32, sigma-scaled).  One batch for, e.g., the
+        frozen-phonon configurations of one probe position."""
+        torch = _torch()
+        B = psi.shape[0]
+        _need(psi, (B, self.ny, self.nx), torch.complex64, "psi")
+        if len(Vs) != B:
+            raise ValueError("one potential stack per probe")
+        for V in Vs:
+            _need(V, (V.shape[0], self.ny, self.nx), torch.float32, "V")
+        arrV = (ctypes.c_void_p * B)(*[V.data_ptr() for V in Vs])
+        S = (ctypes.c_int * B)(*[V.shape[0] for V in Vs])
+        self._call("em_multislice_forward_multi", arrV, S, _ptr(psi), B)
+        return psi
+
     def multislice_adjoint(self, ckpt, gPsi, V=None, t=None, gt=None, gV=None, gpsi0=None, every=1):
         """Gradient w.r.t. the transmission of a loss whose exit-wave gradient is
         gPsi (dL/dRe + i dL/dIm; overwritten).  Returns gt [S][ny][nx]."""

@@ -243,6 +243,16 @@ em_status em_fft2d(em_plan plan, void* data_dev, int nbatch, int dir);
  * em_multislice_adjoint; ckpt_every = 1 keeps every slice. */
 em_status em_multislice_forward(em_plan plan, const float* V_dev, const void* t_dev, int S,
                                 void* psi_dev, int nprobe, void* ckpt_dev, int ckpt_every);
+/* Forward multislice where probe b has its own potential stack V_dev[b]
+ * (sigma-scaled FP32 [S[b]][N]) and slice count: psi_dev[B][N] in: incident
+ * waves (real space); out: exit waves (Fourier, unnormalised), as
+ * em_multislice_forward per probe.  All probes share the plan's grid and
+ * propagator and run through shared fused passes (a chunk table gives each
+ * workgroup its probe's potential slice).  For frozen-phonon configurations,
+ * which the reference runs one after another (ED_HRTEM / CBED_CBEI,
+ * single_simulator.cuh:60-80, 160-180); no reference counterpart of its own. */
+em_status em_multislice_forward_multi(em_plan plan, const float* const* V_dev, const int* S,
+                                      void* psi_dev, int nprobe);
 /* Adjoint of em_multislice_forward with respect to the transmission (no
  * reference counterpart).  gPsi_dev[B][N] = dL/dRe(Psi) + i dL/dIm(Psi) of
  * the exit waves (overwritten); ckpt_dev and ckpt_every as given to the

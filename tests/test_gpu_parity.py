@@ -484,6 +484,43 @@ def test_ragged_matches_separate_runs(em, mix):
         p.close()
 
 
+def test_forward_multi_matches_per_probe_runs(em):
+    """em_multislice_forward_multi with a potential stack per probe (stacks of
+    different depth; probes 1 and 2 share one stack, probes 3 and 4 share
+    another) equals em_multislice_forward run probe by probe, and probe 0's
+    exit wave equals the oracle's."""
+    import torch
+    import cudaem_amd.specimen as sp
+    n, dz, e0 = 128, 1.9525, 300.0
+    lens = probe_lens(20.0, c10=10.0)
+    stacks = []
+    with em.Plan(n, n, 3 * 3.905, 3 * 3.905, dz, e0, max_batch=6) as p:
+        for nz in (2, 3, 5):
+            at = sp.srtio3(3, 3, nz)
+            p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+            stacks.append((at, p.potential_slices().clone()))
+        order = [0, 1, 1, 2, 2, 0]
+        Vs = [stacks[k][1] for k in order]
+        xs = np.array([1.0, 3.5, 6.0, 8.5, 10.0, 4.4])
+        ys = np.array([2.0, 9.0, 4.5, 1.5, 7.0, 6.1])
+        psi = p.build_probes(lens, xs, ys)
+        ref = psi.clone()
+        p.multislice_forward_multi(psi, Vs)
+        got = _np(psi)
+        for b in range(len(order)):
+            w = ref[b:b + 1].clone()
+            p.multislice_forward(w, V=Vs[b])
+            assert rel_l2(got[b], _np(w)[0]) < 1e-6
+        P = _np(p.propagator()).astype(np.complex128)
+        torch.cuda.synchronize()
+    at0 = stacks[0][0]
+    o = OracleSpecimen(at0, n, n, at0["lx"], at0["ly"], dz, e0, O.BORDER_MIN)
+    psi0 = O.probe(n, n, at0["lx"], at0["ly"], o.lam, lens, xs[0], ys[0], 0.0)
+    Psi_o = O.multislice(o.propagator(), psi0, V=o.potentials())
+    assert rel_l2(P, o.propagator()) < TOL_OP
+    assert rel_l2(got[0], Psi_o) < TOL_WAVE
+
+
 def test_adjoint_recompute_and_potential_gradient(em):
     """Checkpointing every 3rd slice with recompute gives the same gradient as
     keeping every slice; the potential gradient is Re(conj(gt) i t)."""
