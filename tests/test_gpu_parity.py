@@ -646,6 +646,47 @@ def test_tuning_variants_agree(em):
     assert rel_l2(touts[1], touts[0]) < 1e-6
 
 
+def test_benchmark_size_properties(em):
+    """The benchmark's own configuration (SrTiO3 10 x 10 x 40, 1024^2, 80
+    slices, 300 kV, 25 mrad), checked through properties that hold at any
+    size:
+      * band limit off: every slice step is unitary up to the unnormalised
+        FFT, so sum |Psi_exit|^2 = N sum |psi_in|^2 (FP32 accumulation);
+      * band limit on: strictly less;
+      * the batched STEM path (graph-captured groups, band skip, fused
+        detector pass) equals the plain forward + detector-sum path."""
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(10, 10, 40)
+    n, dz, e0 = 1024, 1.9525, 300.0
+    lens = probe_lens(25.0)
+    xs = np.array([3.1, 19.5, 33.3, 12.7])
+    ys = np.array([7.7, 19.5, 2.4, 30.1])
+    inner, outer = np.array([60.0, 10.0]), np.array([200.0, 40.0])
+    norms = {}
+    for bwl in (False, True):
+        with em.Plan(n, n, at["lx"], at["ly"], dz, e0, bwl=bwl, max_batch=4) as p:
+            S = p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+            assert S == 80
+            zs, ze, _, _ = p.slice_info(0)
+            V = p.potential_slices()
+            psi = p.build_probes(lens, xs, ys, defocus_shift=0.5 * (zs + ze))
+            n_in = (psi.abs() ** 2).double().sum(dim=(1, 2))
+            p.multislice_forward(psi, V=V)
+            n_out = (psi.abs() ** 2).double().sum(dim=(1, 2)) / (n * n)
+            norms[bwl] = (_np(n_in), _np(n_out))
+            if bwl:
+                det = _np(p.detector_sums(psi, inner, outer))
+                stem = _np(p.stem(lens, xs, ys, inner, outer))
+        torch.cuda.synchronize()
+    n_in, n_out = norms[False]
+    print("norm ratio (bwl off)", n_out / n_in)
+    assert np.all(np.abs(n_out / n_in - 1) < 2e-4)
+    n_in, n_out = norms[True]
+    assert np.all(n_out < n_in * (1 - 1e-3))
+    assert rel_l2(stem, det) < 1e-5
+
+
 def test_band_skip_is_below_fp32_resolution(em):
     """Skipping the propagator columns below 1e-9 (relative) changes the
     exit waves and the adjoint gradient by less than FP32 resolution; it skips
