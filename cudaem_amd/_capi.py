@@ -35,6 +35,7 @@ EXPORTS = [
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_forward_multi", "em_multislice_adjoint",
     "em_detector_sums", "em_ctf_image", "em_pctf_image", "em_tcc_image", "em_diffraction", "em_stem", "em_thickness",
+    "em_thickness_multi",
     "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_quadrature", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
     "em_plan_set_band_skip",
@@ -119,6 +120,7 @@ _SIGS = {
     "em_diffraction": ([_P, _P, _I, _P, _F, _I], _I),
     "em_stem": ([_P, ctypes.POINTER(em_lens), _I, _PD, _PD, _I, _PD, _PD, _F, _I, _P], _I),
     "em_thickness": ([_P, _P, _I, _P, _I, _D, _D, _I, _F, _I, _P, _PI], _I),
+    "em_thickness_multi": ([_P, _P, _I, _P, _I, _D, _D, _I, _F, _I, _P, _PI], _I),
     "em_multislice_ragged": ([_I, ctypes.POINTER(_P), ctypes.POINTER(_P), _PI, ctypes.POINTER(_P), _PI,
                               _I, _PD, _PD, _I, ctypes.POINTER(_P), _P], _I),
     "em_time_passes": ([_P, _P, _P, _I, _I, ctypes.POINTER(_F), ctypes.POINTER(_F)], _I),

@@ -281,6 +281,40 @@ unsigned row_t_mode(em_plan p, unsigned m) {
   return (m == (ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT) && p->tune.row_variant == 1) ? (m | ROW_T_STREAM) : m;
 }
 
+// Thickness series (run_thickness_simu, thickness_simulator.cuh:11-85): the
+// slice loop with the band-passed intensity recorded after every `stride`-th
+// slice and after the last one.  row_step(s) runs slice s's row pass on wv.
+template <typename RowStep>
+em_status thickness_loop(em_plan p, int S, float2* wv, int nprobe, double inner, double outer, int stride, float w,
+                         int accumulate, float* frames, int* n_frames, RowStep&& row_step) {
+  const long long N = p->N();
+  EM_HIP(p, p->tmp.ensure(sizeof(float2) * N * nprobe));
+  int nf = 0;
+  for (int s = 0; s < S; ++s) {
+    EM_TRY(row_step(s));
+    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, wv, wv, nprobe)));
+    const bool rec = stride <= 1 || ((s + 1) % stride == 0) || s == S - 1;
+    if (!rec) continue;
+    ColArgs c = col_args(p, wv, p->tmp.as<float2>(), nprobe);
+    c.fg2min = mrad_to_g2(inner, p->k.lambda, false);
+    c.fg2max = mrad_to_g2(outer, p->k.lambda, true);
+    // the wave here is in the (y, kx) layout, F_x psi / nx (the unnormalised
+    // inverse of the next row pass restores psi), so the band-pass scale 1/N
+    // of Filter_wave becomes nx/N = 1/ny
+    c.fscale = (float)(1.0 / (double)p->grid.ny);
+    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_FILTER | COL_POST_IFFT | COL_STORE, c));
+    RowArgs ro = row_args(p, p->tmp.as<float2>(), nullptr, nprobe);
+    ro.o1 = frames + (long long)nf * nprobe * N;
+    ro.wgt = w;
+    EM_TRY(row(p, ROW_PRE_IFFT | ROW_OUT_ABS2 | (accumulate ? ROW_OUT_ACCUM : 0u), ro));
+    ++nf;
+  }
+  // back to real space
+  EM_TRY(row(p, ROW_PRE_IFFT, row_args(p, wv, wv, nprobe)));
+  if (n_frames) *n_frames = nf;
+  return EM_OK;
+}
+
 // Fused forward multislice over nb probes at wave (real space in, Fourier out).
 em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
                         float2* ckpt, long long ckpt_slice_stride, int every, bool in_mixed_layout) {
@@ -1255,33 +1289,49 @@ em_status em_thickness(em_plan p, const float* V, int S, void* psi, int nprobe, 
   if (!p || !V || !psi || S <= 0 || nprobe <= 0 || !frames) return EM_ERR_INVALID;
   const long long N = p->N();
   float2* wv = reinterpret_cast<float2*>(psi);
-  EM_HIP(p, p->tmp.ensure(sizeof(float2) * N * nprobe));
-  int nf = 0;
-  for (int s = 0; s < S; ++s) {
+  return thickness_loop(p, S, wv, nprobe, inner, outer, stride, w, accumulate, frames, n_frames, [&](int s) {
     RowArgs r = row_args(p, wv, wv, nprobe);
     r.V = V + (long long)s * N;
-    EM_TRY(row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, r));
-    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, col_args(p, wv, wv, nprobe)));
-    const bool rec = stride <= 1 || ((s + 1) % stride == 0) || s == S - 1;
-    if (!rec) continue;
-    ColArgs c = col_args(p, wv, p->tmp.as<float2>(), nprobe);
-    c.fg2min = mrad_to_g2(inner, p->k.lambda, false);
-    c.fg2max = mrad_to_g2(outer, p->k.lambda, true);
-    // the wave here is in the (y, kx) layout, F_x psi / nx (the unnormalised
-    // inverse of the next row pass restores psi), so the band-pass scale 1/N
-    // of Filter_wave becomes nx/N = 1/ny
-    c.fscale = (float)(1.0 / (double)p->grid.ny);
-    EM_TRY(col(p, COL_PRE_FFT | COL_MUL_FILTER | COL_POST_IFFT | COL_STORE, c));
-    RowArgs ro = row_args(p, p->tmp.as<float2>(), nullptr, nprobe);
-    ro.o1 = frames + (long long)nf * nprobe * N;
-    ro.wgt = w;
-    EM_TRY(row(p, ROW_PRE_IFFT | ROW_OUT_ABS2 | (accumulate ? ROW_OUT_ACCUM : 0u), ro));
-    ++nf;
-  }
-  // back to real space
-  EM_TRY(row(p, ROW_PRE_IFFT, row_args(p, wv, wv, nprobe)));
-  if (n_frames) *n_frames = nf;
-  return EM_OK;
+    return row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT, r);
+  });
+}
+
+em_status em_thickness_multi(em_plan p, const float* const* V, int S, void* psi, int nprobe, double inner,
+                             double outer, int stride, float w, int accumulate, float* frames, int* n_frames) {
+  if (!p || !V || !psi || S <= 0 || nprobe <= 0 || !frames) return EM_ERR_INVALID;
+  for (int b = 0; b < nprobe; ++b)
+    if (!V[b]) return EM_ERR_INVALID;
+  const long long N = p->N();
+  float2* wv = reinterpret_cast<float2*>(psi);
+  // chunk table of every slice's row pass: runs of consecutive probes that
+  // share one potential stack, each with that stack's slice
+  const int pb = row_pb(p, nprobe);
+  std::vector<PassChunk> chunks;
+  for (int s = 0; s < S; ++s)
+    for (int b = 0; b < nprobe;) {
+      int n = 1;
+      while (b + n < nprobe && n < pb && V[b + n] == V[b]) ++n;
+      PassChunk ch{};
+      ch.out = wv + (long long)b * N;
+      ch.in = ch.out;
+      ch.table = V[b] + (long long)s * N;
+      ch.n = n;
+      chunks.push_back(ch);
+      b += n;
+    }
+  const int per_slice = (int)chunks.size() / S;
+  if (per_slice > 65535) return p->fail(EM_ERR_INVALID, "too many probes in one pass");
+  // an earlier call's launches may still be reading the table
+  EM_HIP(p, hipStreamSynchronize(p->stream));
+  EM_HIP(p, p->mchunks.ensure(sizeof(PassChunk) * chunks.size()));
+  EM_HIP(p, hipMemcpy(p->mchunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
+  const PassChunk* dchunks = p->mchunks.as<PassChunk>();
+  return thickness_loop(p, S, wv, nprobe, inner, outer, stride, w, accumulate, frames, n_frames, [&](int s) {
+    RowArgs r = row_args(p, nullptr, nullptr, per_slice);
+    r.pb = 1;
+    r.chunks = dchunks + (long long)s * per_slice;
+    return row(p, (s > 0 ? ROW_PRE_IFFT : 0u) | ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS, r);
+  });
 }
 
 em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, const int* S, void* const* psi,

@@ -425,29 +425,69 @@ void run_thickness_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_D
   o->row = m.ny;
   o->col = m.nx;
   const em_lens cond = lens_of(m.cond_lens);
-  Dev psi(N * 8);
-  std::unique_ptr<Dev> frames;
+  // Frozen-phonon configurations with one slice count run as the waves of one
+  // batch (em_thickness_multi), in blocks as in run_simu; their frames are
+  // summed into the output in configuration order.
+  constexpr int kMaxConfBlock = 64;
+  constexpr size_t kConfBlockBytes = size_t(8) << 30;
+  const int kmax = std::min(nconf, kMaxConfBlock);
+  Dev psi(N * 8 * (size_t)kmax);
+  std::unique_ptr<Dev> frames, scratch;
+  size_t scratch_bytes = 0;
   int nf_max = 0, nf = 0;
-  for (int iconf = 1; iconf <= nconf; ++iconf) {
-    const Atom_Coord_Data<T> at = atoms_of_config(m, iconf);
-    const int S = set_specimen(plan.p, at, EM_BORDER_MIN);
-    if (!frames) {
-      nf_max = S;
-      frames.reset(new Dev(sizeof(float) * N * nf_max));
-      hip_ok(hipMemset(frames->p, 0, sizeof(float) * N * nf_max), "memset");
+  const int st = m.thickness_stride;
+  auto frames_of = [st](int S) { return st <= 1 ? S : S / st + (S % st != 0 ? 1 : 0); };
+  for (int iconf = 1; iconf <= nconf;) {
+    std::vector<std::unique_ptr<Dev>> V;
+    std::vector<const float*> Vp;
+    size_t used = 0;
+    int S0 = 0;
+    while (iconf <= nconf && (int)V.size() < kmax && used < kConfBlockBytes) {
+      const Atom_Coord_Data<T> at = atoms_of_config(m, iconf);
+      const int S = set_specimen(plan.p, at, EM_BORDER_MIN);
+      if (!frames) {
+        nf_max = S;
+        frames.reset(new Dev(sizeof(float) * N * nf_max));
+        hip_ok(hipMemset(frames->p, 0, sizeof(float) * N * nf_max), "memset");
+      }
+      if (S > nf_max) throw simulation_error("slice count changed between phonon configurations");
+      if (!V.empty() && S != S0) break;  // this configuration opens the next block
+      S0 = S;
+      V.push_back(std::make_unique<Dev>((size_t)S * N * 4));
+      ok(em_potential_slices(plan.p, 0, S, V.back()->as<float>()), plan.p, "em_potential_slices");
+      Vp.push_back(V.back()->as<float>());
+      used += ((size_t)S + (size_t)frames_of(S)) * N * 4;  // potentials and this configuration's frames
+      void* slot = psi.as<float2>() + (long long)(V.size() - 1) * (long long)N;
+      if (m.iw_type == eIWT_Convergent_Wave) {
+        const double xb = m.incident_wave_x_pos, yb = m.incident_wave_y_pos;
+        ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, slot), plan.p, "em_build_probes");
+      } else {
+        ok(em_plane_wave(plan.p, 1, slot), plan.p, "em_plane_wave");
+      }
+      ++iconf;
     }
-    if (S > nf_max) throw simulation_error("slice count changed between phonon configurations");
-    Dev V((size_t)S * N * 4);
-    ok(em_potential_slices(plan.p, 0, S, V.as<float>()), plan.p, "em_potential_slices");
-    if (m.iw_type == eIWT_Convergent_Wave) {
-      const double xb = m.incident_wave_x_pos, yb = m.incident_wave_y_pos;
-      ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, psi.p), plan.p, "em_build_probes");
+    const int k = (int)V.size();
+    const double inner = m.detector.ang_inner[m.thickness_det], outer = m.detector.ang_outer[m.thickness_det];
+    if (k == 1) {
+      ok(em_thickness(plan.p, Vp[0], S0, psi.p, 1, inner, outer, m.thickness_stride, w, 1, frames->as<float>(), &nf),
+         plan.p, "em_thickness");
     } else {
-      ok(em_plane_wave(plan.p, 1, psi.p), plan.p, "em_plane_wave");
+      const size_t need = sizeof(float) * N * (size_t)frames_of(S0) * (size_t)k;
+      if (need > scratch_bytes) {
+        scratch.reset();
+        scratch.reset(new Dev(need));
+        scratch_bytes = need;
+      }
+      ok(em_thickness_multi(plan.p, Vp.data(), S0, psi.p, k, inner, outer, m.thickness_stride, w, 0,
+                            scratch->as<float>(), &nf),
+         plan.p, "em_thickness_multi");
+      // frames[f] += sum over the block's configurations (a deterministic reduction)
+      for (int f = 0; f < nf; ++f)
+        hip_ok(em::launch_grad_reduce((long long)N / 2, k,
+                                      reinterpret_cast<const float2*>(scratch->as<float>() + (size_t)f * k * N),
+                                      reinterpret_cast<float2*>(frames->as<float>() + (size_t)f * N), 1, nullptr),
+               "frame sum");
     }
-    ok(em_thickness(plan.p, V.as<float>(), S, psi.p, 1, m.detector.ang_inner[m.thickness_det],
-                    m.detector.ang_outer[m.thickness_det], m.thickness_stride, w, 1, frames->as<float>(), &nf),
-       plan.p, "em_thickness");
     hip_ok(hipDeviceSynchronize(), "sync");
   }
   std::vector<float> h(N * nf);

@@ -440,6 +440,29 @@ class Plan:
                    float(w), 1 if accumulate else 0, _ptr(frames), ctypes.byref(nf))
         return frames[:nf.value]
 
+    def thickness_multi(self, Vs, psi, inner_mrad, outer_mrad, stride=1, w=1.0, frames=None, accumulate=False):
+        """thickness() with a potential stack per wave: Vs[b] is [S][ny][nx]
+        (one S for all), e.g. the frozen-phonon configurations of one probe."""
+        torch = _torch()
+        B = psi.shape[0]
+        if len(Vs) != B or B == 0:
+            raise ValueError("one potential stack per wave")
+        S = Vs[0].shape[0]
+        for V in Vs:
+            _need(V, (S, self.ny, self.nx), torch.float32, "V")
+        _need(psi, (B, self.ny, self.nx), torch.complex64, "psi")
+        nf_max = S if stride <= 1 else (S + stride - 1) // stride + 1
+        if frames is None:
+            frames = torch.zeros((nf_max, B, self.ny, self.nx), dtype=torch.float32, device="cuda")
+        if frames.shape[0] < nf_max:
+            raise ValueError("frames too small")
+        _need(frames, (frames.shape[0], B, self.ny, self.nx), torch.float32, "frames")
+        arrV = (ctypes.c_void_p * B)(*[V.data_ptr() for V in Vs])
+        nf = ctypes.c_int()
+        self._call("em_thickness_multi", arrV, S, _ptr(psi), B, float(inner_mrad), float(outer_mrad), int(stride),
+                   float(w), 1 if accumulate else 0, _ptr(frames), ctypes.byref(nf))
+        return frames[:nf.value]
+
 
 def ragged(plans, Vs, psis, inner_mrad, outer_mrad, stride):
     """One graph launch over requests on different grids (em_multislice_ragged).

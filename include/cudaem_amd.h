@@ -318,6 +318,15 @@ em_status em_thickness(em_plan plan, const float* V_dev, int S, void* psi_dev, i
                        double inner_mrad, double outer_mrad, int stride, float w, int accumulate,
                        float* frames_dev, int* n_frames);
 
+/* em_thickness with a potential stack per wave (V_dev[b], sigma-scaled FP32
+ * [S][N], one slice count S for all), the row passes sharing one launch over
+ * all waves: frozen-phonon configurations of run_thickness_simu
+ * (thickness_simulator.cuh:11-85), which the reference runs one after
+ * another.  frames_dev[nf][B][N] as em_thickness. */
+em_status em_thickness_multi(em_plan plan, const float* const* V_dev, int S, void* psi_dev, int nprobe,
+                             double inner_mrad, double outer_mrad, int stride, float w, int accumulate,
+                             float* frames_dev, int* n_frames);
+
 /* Ragged batch: requests on different grids run as one HIP graph launch.
  * Requests of one grid size (and band-limit skip range) step through their
  * slices together: each row and propagator column pass of a slice step is one

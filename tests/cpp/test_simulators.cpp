@@ -177,6 +177,22 @@ int main(int argc, char** argv) {
       write(out + "/thickness.f32", o.thickness_frames);
       std::printf("thickness %zu\n", o.thickness_frames.size());
     }
+    {  // thickness series with three frozen-phonon configurations (one batch)
+      multislice_params<float> p = base(128, 3, 4);
+      p.simulation_type = eTEMST_Thickness;
+      p.iw_type = eIWT_Convergent_Wave;
+      p.incident_wave_x_pos = 2.f;
+      p.incident_wave_y_pos = 3.f;
+      p.thickness_stride = 3;
+      p.phonon_model = ePM_Frozen_Phonon;
+      p.pn_nconf = 3;
+      p.pn_seed = 91;
+      p.pn_sigma = 0.08f;
+      output_Multi_Data<float> o;
+      sim.executionThickness(p, &o);
+      write(out + "/thickness_fp.f32", o.thickness_frames);
+      std::printf("thickness_fp %zu\n", o.thickness_frames.size());
+    }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

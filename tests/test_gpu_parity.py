@@ -521,6 +521,33 @@ def test_forward_multi_matches_per_probe_runs(em):
     assert rel_l2(got[0], Psi_o) < TOL_WAVE
 
 
+def test_thickness_multi_matches_per_wave_runs(em):
+    """em_thickness_multi (a potential stack per wave; waves 1 and 2 share
+    one) gives each wave the frames and final wave of em_thickness."""
+    import torch
+    import cudaem_amd.specimen as sp
+    n, dz, e0 = 128, 1.9525, 200.0
+    lens = probe_lens(18.0)
+    at0 = sp.srtio3(3, 3, 3)
+    with em.Plan(n, n, at0["lx"], at0["ly"], dz, e0, max_batch=4) as p:
+        stacks = []
+        for iconf in (1, 2, 3):
+            at = sp.vibrate(at0, 0.1, 5, iconf, dims=(True, True, False))
+            p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+            stacks.append(p.potential_slices().clone())
+        Vs = [stacks[0], stacks[1], stacks[1], stacks[2]]
+        psi = p.build_probes(lens, [2.0, 5.0, 7.5, 9.0], [3.0, 1.0, 6.5, 8.0])
+        ref = psi.clone()
+        fr = _np(p.thickness_multi(Vs, psi, 30.0, 90.0, stride=2, w=0.5))
+        for b in range(4):
+            w = ref[b:b + 1].clone()
+            f1 = _np(p.thickness(Vs[b], w, 30.0, 90.0, stride=2, w=0.5))
+            assert fr.shape[0] == f1.shape[0]
+            assert rel_l2(fr[:, b], f1[:, 0]) < 1e-6
+            assert rel_l2(_np(psi)[b], _np(w)[0]) < 1e-6
+        torch.cuda.synchronize()
+
+
 def test_adjoint_recompute_and_potential_gradient(em):
     """Checkpointing every 3rd slice with recompute gives the same gradient as
     keeping every slice; the potential gradient is Re(conj(gt) i t)."""

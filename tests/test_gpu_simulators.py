@@ -155,6 +155,25 @@ def test_cpp_stem_general(outputs):
     assert rel_l2(got, ref) < 1e-3
 
 
+def test_cpp_thickness_frozen_phonons(outputs):
+    """Three frozen-phonon configurations (run as one batch of waves, each
+    with its own potentials) against the oracle's weighted sum of series."""
+    at = atoms_f32()
+    acc = None
+    for iconf in (1, 2, 3):
+        v = sp.vibrate(at, np.float32(0.08).item(), 91, iconf, dims=(True, True, False))
+        v = {k: (np.asarray(val, dtype=np.float32).astype(np.float64) if k in ("x", "y", "z") else val)
+             for k, val in v.items()}
+        o = OracleSpecimen(v, N, N, L, L, DZ, E0, O.BORDER_MIN)
+        psi = O.probe(N, N, L, L, o.lam, COND, 2.0, 3.0, o.zmid0())
+        g2min, g2max = (np.sin(40e-3) / o.lam) ** 2, (np.sin(120e-3) / o.lam) ** 2
+        frames, _ = O.thickness(o.propagator(), o.potentials(), psi, L, L, g2min, g2max, stride=3)
+        acc = frames / 3 if acc is None else acc + frames / 3
+    got = outputs("thickness_fp.f32").reshape(-1, N, N)
+    assert got.shape == acc.shape
+    assert rel_l2(got, acc) < 1e-3
+
+
 def test_cpp_thickness(outputs):
     at = atoms_f32()
     o = OracleSpecimen(at, N, N, L, L, DZ, E0, O.BORDER_MIN)
