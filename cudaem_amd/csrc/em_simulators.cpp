@@ -201,6 +201,25 @@ void multislice_params<T>::validate_parameters() const {
 }
 
 template <class T>
+T multislice_params<T>::output_lamda() const {
+  double lam = 0, gam = 0, sig = 0;
+  ok(em_host_constants((double)E_0, &lam, &gam, &sig), nullptr, "em_host_constants");
+  return (T)lam;
+}
+template <class T>
+T multislice_params<T>::output_gama() const {
+  double lam = 0, gam = 0, sig = 0;
+  ok(em_host_constants((double)E_0, &lam, &gam, &sig), nullptr, "em_host_constants");
+  return (T)gam;
+}
+template <class T>
+T multislice_params<T>::output_sigma() const {
+  double lam = 0, gam = 0, sig = 0;
+  ok(em_host_constants((double)E_0, &lam, &gam, &sig), nullptr, "em_host_constants");
+  return (T)sig;
+}
+
+template <class T>
 void vibrate_atom_positions(const Atom_Coord_Data<T>& in, Atom_Coord_Data<T>& out, unsigned long long seed,
                             int iconf, T sigma, const bool dims[3]) {
   const uint64_t g = 0x9E3779B97F4A7C15ull;
@@ -303,7 +322,7 @@ void run_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T>* o)
         coh.dsf_npoints = m.obj_lens.dsf_npoints;
         coh.ssf_beta = m.obj_lens.ssf_beta;
         coh.ssf_npoints = m.obj_lens.ssf_npoints;
-        coh.mode = (int)m.temp_spat_incoh;
+        coh.mode = (int)m.temporal_spatial_incoh;
         if (m.illumination_model == eIM_Partial_Coherent)
           ok(em_pctf_image(plan.p, &obj, &coh, wj, 1, I, re, im, w, 1), plan.p, "em_pctf_image");
         else if (m.illumination_model == eIM_Trans_Cross_Coef)

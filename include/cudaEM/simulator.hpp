@@ -101,6 +101,12 @@ struct Element_Potential {
   T cnl[6] = {1, 1, 1, 1, 1, 1};
 };
 
+// A point or direction in 3D (the reference's r3d).
+template <class T>
+struct r3d {
+  T x = 0, y = 0, z = 0;
+};
+
 template <class T>
 struct multislice_params {
   eTEM_Sim_Type simulation_type = eTEMST_HRTEM;
@@ -116,7 +122,8 @@ struct multislice_params {
   eIncident_Wave_Type iw_type = eIWT_Plane_Wave;
   T incident_wave_x_pos = 0, incident_wave_y_pos = 0;
   eIllumination_Model illumination_model = eIM_Coherent;
-  eTemporal_Spatial_Incoh temp_spat_incoh = eTSI_Temporal_Spatial;  // spreads of partial coherence
+  // spreads of partial coherence (host name read at multislice_params_dev.cuh:112)
+  eTemporal_Spatial_Incoh temporal_spatial_incoh = eTSI_Temporal_Spatial;
   // specimen
   Atom_Coord_Data<T> atoms;
   // fine slicing (multislice_params_dev.cuh:23-24): each element's 3D
@@ -147,6 +154,12 @@ struct multislice_params {
   // detector thickness_det (the reference uses detector 1)
   int thickness_stride = 1;
   int thickness_det = 1;
+  // specimen rotation (multislice_params_dev.cuh:35-37, 119-121): carried
+  // like the reference's shipped code carries them; none of its kernels
+  // applies a rotation, and neither does this build
+  int nrot = 1;
+  r3d<T> spec_rot_u0{0, 0, 1};
+  r3d<T> spec_rot_center_p{0, 0, 0};
 
   void assign(const multislice_params& o) { *this = o; }
   void validate_parameters() const;
@@ -158,7 +171,21 @@ struct multislice_params {
   }
   bool frozen_phonon_enable() const { return phonon_model == ePM_Frozen_Phonon; }
   bool scanning_enable() const { return simulation_type == eTEMST_STEM; }
+  // batched probe launches (tile scanning with more than one probe a batch)
+  bool batch_enable() const { return stem_scanning == eTile_scanning && probe_batch > 1; }
   int output_scan_pos_num() const { return STEM_col * STEM_row; }
+  // accessors the reference's device parameters read (multislice_params_dev.cuh:88-199)
+  int output_STEM_Col() const { return STEM_col; }
+  int output_STEM_Row() const { return STEM_row; }
+  unsigned long long output_phonn_seed() const { return pn_seed; }
+  eIncident_Wave_Type output_incident_wave_type() const { return iw_type; }
+  const Atom_Coord_Data<T>& output_Atom_data() const { return atoms; }
+  T output_scan_interval_x() const { return STEM_col > 0 ? (scan_range[2] - scan_range[0]) * lx / STEM_col : T(0); }
+  T output_scan_interval_y() const { return STEM_row > 0 ? (scan_range[3] - scan_range[1]) * ly / STEM_row : T(0); }
+  // relativistic wavelength (A), gamma and interaction constant (rad/(V A)) at E_0
+  T output_lamda() const;
+  T output_gama() const;
+  T output_sigma() const;
 };
 
 template <class T>

@@ -4,6 +4,7 @@
 //
 //   test_simulators vibrate <out_dir>   phonon displacements only (no GPU)
 //   test_simulators io <out_dir>        atoms.txt -> atoms.f32 through the I/O helpers (no GPU)
+//   test_simulators params <out_dir>    parameter accessors -> params.f64 (no GPU)
 //   test_simulators run <out_dir>       HRTEM, ED, CBED, STEM (tile and
 //                                       general scanning), thickness series
 #include <cstdio>
@@ -64,6 +65,29 @@ int main(int argc, char** argv) {
       }
       write(out + "/atoms.f32", r);
       if (read_raw_data<float>(out + "/atoms.f32") != r) throw simulation_error("raw round trip");
+      return 0;
+    }
+    if (mode == "params") {  // the reference's parameter names and accessors (no GPU)
+      multislice_params<double> p;
+      p.E_0 = 300;
+      p.temporal_spatial_incoh = eTSI_Temporal;
+      p.nrot = 1;
+      p.spec_rot_u0 = r3d<double>{0, 0, 1};
+      p.spec_rot_center_p = r3d<double>{1, 2, 3};
+      p.lx = 40;
+      p.ly = 30;
+      p.STEM_col = 8;
+      p.STEM_row = 6;
+      p.set_Scanning({0.f, 0.f, 0.5f, 1.f});
+      p.pn_seed = 42;
+      p.initial_CL();
+      const std::vector<double> r = {p.output_lamda(), p.output_gama(), p.output_sigma(),
+                                     (double)p.output_STEM_Col(), (double)p.output_STEM_Row(),
+                                     p.output_scan_interval_x(), p.output_scan_interval_y(),
+                                     (double)p.output_phonn_seed(),
+                                     p.output_incident_wave_type() == eIWT_Convergent_Wave ? 1.0 : 0.0,
+                                     p.batch_enable() ? 1.0 : 0.0, (double)p.output_scan_pos_num()};
+      write_raw_data(out + "/params.f64", r);
       return 0;
     }
     if (mode == "vibrate") {

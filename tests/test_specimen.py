@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import cudaem_amd.specimen as sp
+import pyoracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -79,6 +80,21 @@ def test_cpp_atom_text_and_raw_io(tmp_path):
                                                [8, 1.9525, 1.9525, 0, 1]], dtype=np.float32))
     (tmp_path / "atoms.txt").write_text("8 1.0 2.0\n")
     assert subprocess.run([exe, "io", str(tmp_path)], capture_output=True).returncode != 0
+
+
+def test_cpp_parameter_names_and_accessors(tmp_path):
+    """multislice_params carries the reference's host names
+    (temporal_spatial_incoh, nrot, spec_rot_u0, spec_rot_center_p) and its
+    accessors: output_lamda / output_gama / output_sigma equal the oracle's
+    constants; scan intervals, counts, seed, wave type and batching."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_simulators")
+    if not os.path.exists(exe):
+        pytest.skip("C++ test program not built")
+    subprocess.run([exe, "params", str(tmp_path)], check=True)
+    r = np.fromfile(tmp_path / "params.f64", dtype=np.float64)
+    lam, gam, sig = O.constants(300.0)
+    np.testing.assert_allclose(r[:3], [lam, gam, sig], rtol=1e-12)
+    np.testing.assert_allclose(r[3:], [8, 6, 0.5 * 40 / 8, 30 / 6, 42, 1, 1, 48])
 
 
 def test_raster_and_cut():
