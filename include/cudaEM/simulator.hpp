@@ -23,6 +23,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace cudaEM {
@@ -63,7 +64,32 @@ struct Atom_Coord_Data {
     z.push_back(zi);
     occ.push_back(occi);
   }
+  void assign(const Atom_Coord_Data& o) { *this = o; }
+  // translate so that (x0, y0) becomes the origin (scan_simulator.cuh:65, 157)
+  void shift_atoms(T x0, T y0) {
+    for (T& v : x) v -= x0;
+    for (T& v : y) v -= y0;
+  }
 };
+
+// Keep the atoms inside the closed box [x0, x1] x [y0, y1]; the scanners pass
+// their window widened by 2 A on every side (scan_simulator.cuh:61-62, 153-154).
+template <class T>
+void remove_atoms_outside_xy_range(Atom_Coord_Data<T>& a, T x0, T x1, T y0, T y1) {
+  Atom_Coord_Data<T> k;
+  k.x_l = a.x_l;
+  k.y_l = a.y_l;
+  const bool has_occ = a.occ.size() == a.size();
+  for (std::size_t i = 0; i < a.size(); ++i)
+    if (a.x[i] >= x0 && a.x[i] <= x1 && a.y[i] >= y0 && a.y[i] <= y1) {
+      k.Z.push_back(a.Z[i]);
+      k.x.push_back(a.x[i]);
+      k.y.push_back(a.y[i]);
+      k.z.push_back(a.z[i]);
+      if (has_occ) k.occ.push_back(a.occ[i]);
+    }
+  a = std::move(k);
+}
 
 // Lens_Dev fields (lens_dev.cuh:12-66): aberrations in Angstrom, angles in
 // rad, apertures as semi-angles in mrad.

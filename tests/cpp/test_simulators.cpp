@@ -5,6 +5,7 @@
 //   test_simulators vibrate <out_dir>   phonon displacements only (no GPU)
 //   test_simulators io <out_dir>        atoms.txt -> atoms.f32 through the I/O helpers (no GPU)
 //   test_simulators params <out_dir>    parameter accessors -> params.f64 (no GPU)
+//   test_simulators cut <out_dir>       atom window cut and shift -> cut.f64 (no GPU)
 //   test_simulators run <out_dir>       HRTEM, ED, CBED, STEM (tile and
 //                                       general scanning), thickness series
 #include <cstdio>
@@ -65,6 +66,22 @@ int main(int argc, char** argv) {
       }
       write(out + "/atoms.f32", r);
       if (read_raw_data<float>(out + "/atoms.f32") != r) throw simulation_error("raw round trip");
+      return 0;
+    }
+    if (mode == "cut") {  // remove_atoms_outside_xy_range + shift_atoms (no GPU)
+      Atom_Coord_Data<double> at;
+      for (int i = 0; i < 40; ++i) at.push_back(6 + i % 3, -3.0 + 0.5 * i, 7.0 - 0.35 * i, 0.1 * i, 0.5 + 0.01 * i);
+      remove_atoms_outside_xy_range(at, -1.0 - 2.0, 9.0 + 2.0, -2.0 - 2.0, 4.0 + 2.0);
+      at.shift_atoms(-1.0, -2.0);
+      std::vector<double> r;
+      for (size_t i = 0; i < at.size(); ++i) {
+        r.push_back(at.Z[i]);
+        r.push_back(at.x[i]);
+        r.push_back(at.y[i]);
+        r.push_back(at.z[i]);
+        r.push_back(at.occ[i]);
+      }
+      write_raw_data(out + "/cut.f64", r);
       return 0;
     }
     if (mode == "params") {  // the reference's parameter names and accessors (no GPU)

@@ -97,6 +97,22 @@ def test_cpp_parameter_names_and_accessors(tmp_path):
     np.testing.assert_allclose(r[3:], [8, 6, 0.5 * 40 / 8, 30 / 6, 42, 1, 1, 48])
 
 
+def test_cpp_atom_cut_matches_python(tmp_path):
+    """remove_atoms_outside_xy_range + shift_atoms of the C++ surface keep and
+    move the same atoms as cut_window (closed box, 2 A margin)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_simulators")
+    if not os.path.exists(exe):
+        pytest.skip("C++ test program not built")
+    subprocess.run([exe, "cut", str(tmp_path)], check=True)
+    r = np.fromfile(tmp_path / "cut.f64", dtype=np.float64).reshape(-1, 5)
+    i = np.arange(40)
+    at = {"Z": 6 + i % 3, "x": -3.0 + 0.5 * i, "y": 7.0 - 0.35 * i, "z": 0.1 * i, "occ": 0.5 + 0.01 * i}
+    c = sp.cut_window(at, -1.0, 9.0, -2.0, 4.0)
+    assert 0 < len(c["x"]) < 40
+    np.testing.assert_array_equal(r[:, 0], c["Z"])
+    np.testing.assert_allclose(r[:, 1:], np.stack([c["x"], c["y"], c["z"], c["occ"]], axis=1), rtol=0, atol=1e-12)
+
+
 def test_raster_and_cut():
     x, y = sp.raster(4, 3, 8.0, 6.0)
     assert len(x) == 12 and x[1] == 2.0 and y[4] == 2.0
