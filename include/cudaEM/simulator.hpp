@@ -36,6 +36,8 @@ enum eTemporal_Spatial_Incoh { eTSI_Temporal_Spatial = 1, eTSI_Temporal = 2, eTS
 enum eScanning_Type { eTile_scanning = 1, eGeneral_scanning = 2 };
 enum eData_Record { eIntensity = 1, eWave = 2 };
 enum eMatch_Border { eMB_Min = 0, eMB_Max = 1, eMB_Middle = 2 };
+// output space of ED_HRTEM (single_simulator.cuh:12, 32, 103)
+enum eSpace { eS_Real = 1, eS_Reciprocal = 2 };
 // 3D potential parameterisations of fine slicing (Vr_dVrir_dev,
 // basic_functions_dev.cuh:189-214); values as EM_PT_* of cudaem_amd.h.
 enum ePotential_Type {
