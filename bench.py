@@ -287,23 +287,27 @@ def main():
     lens = {"outer_aper_mrad": ALPHA}
     out = torch.zeros((1, npos), dtype=torch.float32, device="cuda")
 
-    def step(iconf):
+    def step(iconf, w, accumulate):
         at = em.specimen.vibrate(at0, U_RMS, SEED, iconf)
         S = plan.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
-        plan.stem(lens, xs, ys, DET[0], DET[1], w=1.0, out=out, accumulate=False)
+        plan.stem(lens, xs, ys, DET[0], DET[1], w=w, out=out, accumulate=accumulate)
         return S
 
     for i in range(args.warmup):
-        step(1 + i)
+        step(1 + i, 1.0, False)
     torch.cuda.synchronize()
+    # the timed steps are the frozen-phonon configurations 1..steps of one
+    # image: each adds its detector sums with weight 1/steps
+    w = 1.0 / args.steps
+    out.zero_()
     t0 = time.perf_counter()
     S = 0
     sums = None
     for i in range(args.steps):
-        S = step(1 + i)
+        S = step(1 + i, w, True)
         if i == 0:
             torch.cuda.synchronize()
-            sums = out[0].cpu().numpy().copy()
+            sums = out[0].cpu().numpy().copy() / w  # configuration 1 alone, for the oracle check
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ps = args.steps * npos * S / dt
@@ -368,7 +372,8 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": "batched STEM, SrTiO3 10x10x40 cells, 1024^2 grid, 80 slices, 300 kV, "
-                               "25 mrad, 64x64 scan, 1 frozen-phonon configuration per step",
+                               "25 mrad, 64x64 scan, 1 frozen-phonon configuration per step, the steps' detector "
+                               "sums averaged into one image (weight 1/steps)",
                    "grid": N_GRID, "slices": S, "probes_per_step": npos, "batch": args.batch,
                    "probe_group": G, "tuning": tun, "autotune_ms_per_probe_slice": tune_ms,
                    "phonon_u_rms": U_RMS, "parallelism": "single GPU"},
