@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--group", type=int, default=0, help="probes kept together per fused pass (0 = auto)")
-    ap.add_argument("--cpu-probes", type=int, default=4)
+    ap.add_argument("--cpu-probes", type=int, default=16, help="probes of the CPU baseline (one thread each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--npos", type=int, default=0, help="scan positions per step (0 = the full 64x64 raster)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -93,7 +93,8 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
                        g2max, nthreads=threads)
     dt = time.perf_counter() - t0
     rel = float(np.linalg.norm(ngpu_sums[:n_probe] - ref[0]) / np.linalg.norm(ref[0]))
-    return n_probe * S / dt, used, rel, n_probe * S
+    # threads with work: the oracle runs one probe per thread
+    return n_probe * S / dt, min(used, n_probe), rel, n_probe * S
 
 
 def plain_path(em, torch, at, xs, ys, n_probe):
