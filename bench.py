@@ -93,8 +93,14 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
                        g2max, nthreads=threads)
     dt = time.perf_counter() - t0
     rel = float(np.linalg.norm(ngpu_sums[:n_probe] - ref[0]) / np.linalg.norm(ref[0]))
+    # the same loop in FP32 (the GPU's arithmetic), timed beside it
+    t0 = time.perf_counter()
+    r32, _ = O.stem(n, n, at["lx"], at["ly"], lam, lens, zmid0, Po, Vo, xs[:n_probe], ys[:n_probe], g2min,
+                    g2max, nthreads=threads, fp32=True)
+    dt32 = time.perf_counter() - t0
+    fp32 = {"value": n_probe * S / dt32, "rel_l2_vs_fp64": float(np.linalg.norm(r32[0] - ref[0]) / np.linalg.norm(ref[0]))}
     # threads with work: the oracle runs one probe per thread
-    return n_probe * S / dt, min(used, n_probe), rel, n_probe * S
+    return n_probe * S / dt, min(used, n_probe), rel, n_probe * S, fp32
 
 
 def plain_path(em, torch, at, xs, ys, n_probe):
@@ -353,11 +359,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         at1 = em.specimen.vibrate(at0, U_RMS, SEED, 1)
-        v, used, rel, units = cpu_baseline(at1, xs, ys, sums, args.cpu_probes)
+        v, used, rel, units, fp32 = cpu_baseline(at1, xs, ys, sums, args.cpu_probes)
         cpu = {"value": v, "unit": "probe-slices/s", "cores": used, "kind": "port",
                "sample": f"{args.cpu_probes} probes x {S} slices of phonon configuration 1 "
                          f"({units} probe-slices, FP64 host oracle, OpenMP over probes)",
-               "gpu_rel_l2_vs_oracle": rel}
+               "gpu_rel_l2_vs_oracle": rel,
+               "fp32_variant": fp32}
 
     line = {
         "metric": "probe-slices/s at 1024^2 FP32, % HBM roofline; batched/plain and adjoint/fwd ratio",

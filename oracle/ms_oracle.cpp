@@ -5,6 +5,9 @@
 // by nothing else.  The product library (cudaem_amd/) never links, loads or
 // calls it, and never falls back to it.
 //
+// orc_stem_f32 runs the STEM loop in FP32 as a second CPU timing; it is
+// checked against the FP64 loop and is not an oracle.
+//
 // Every function restates the reference's algorithm in double precision and
 // cites the reference file:line it follows (the reference is a flat directory
 // of CUDA headers).  Pieces the reference does not ship (element tables, the
@@ -361,13 +364,17 @@ void fine_spline(int pt, const double* cl, const double* cnl, double a, double d
 // Mixed-radix FFT, unnormalised, sign -1 forward / +1 inverse.
 // Recursive decimation in time over the prime factors of n (4 preferred).
 // ---------------------------------------------------------------------------
-struct Fft {
+// Mixed-radix recursive FFT in the real type R (double: the oracle; float:
+// the FP32 variant timed beside it).  Twiddles are formed in double.
+template <class R>
+struct FftT {
+  typedef std::complex<R> C;
   int n;
-  std::vector<cd> w;  // w[j] = exp(-2 pi i j / n)
-  explicit Fft(int n_) : n(n_), w(n_) {
+  std::vector<C> w;  // w[j] = exp(-2 pi i j / n)
+  explicit FftT(int n_) : n(n_), w(n_) {
     for (int j = 0; j < n; ++j) {
       const double ang = -2.0 * kPi * (double)j / (double)n;
-      w[j] = cd(std::cos(ang), std::sin(ang));
+      w[j] = C((R)std::cos(ang), (R)std::sin(ang));
     }
   }
   static int first_factor(int m) {
@@ -378,7 +385,7 @@ struct Fft {
     return m;
   }
   // out[k], k < m, = sum_j in[j*stride] exp(sign 2 pi i jk/m); m divides n
-  void rec(const cd* in, int stride, cd* out, int m, int sign, cd* tmp) const {
+  void rec(const C* in, int stride, C* out, int m, int sign, C* tmp) const {
     if (m == 1) { out[0] = in[0]; return; }
     const int p = first_factor(m);
     const int q = m / p;
@@ -387,11 +394,11 @@ struct Fft {
     for (int k = 0; k < q; ++k) {
       for (int r = 0; r < p; ++r) {
         const int e = (int)(((long long)r * k) % m) * step;
-        const cd tw = sign < 0 ? w[e] : std::conj(w[e]);
+        const C tw = sign < 0 ? w[e] : std::conj(w[e]);
         tmp[r] = out[r * q + k] * tw;
       }
       for (int s = 0; s < p; ++s) {
-        cd acc(0.0, 0.0);
+        C acc(0, 0);
         for (int r = 0; r < p; ++r) {
           const int e = (int)(((long long)r * s * q) % m) * step;
           acc += tmp[r] * (sign < 0 ? w[e] : std::conj(w[e]));
@@ -400,18 +407,20 @@ struct Fft {
       }
     }
   }
-  void run(cd* data, int stride, int sign, cd* buf_in, cd* buf_out, cd* tmp) const {
+  void run(C* data, int stride, int sign, C* buf_in, C* buf_out, C* tmp) const {
     for (int j = 0; j < n; ++j) buf_in[j] = data[(long long)j * stride];
     rec(buf_in, 1, buf_out, n, sign, tmp);
     for (int j = 0; j < n; ++j) data[(long long)j * stride] = buf_out[j];
   }
 };
+typedef FftT<double> Fft;
 
 // In-place 2D FFT of a [ny][nx] array: rows, then columns.
-void fft2(cd* a, int nx, int ny, int sign) {
-  Fft fx(nx), fy(ny);
+template <class R>
+void fft2(std::complex<R>* a, int nx, int ny, int sign) {
+  FftT<R> fx(nx), fy(ny);
   const int mx = std::max(nx, ny);
-  std::vector<cd> bi(mx), bo(mx), tmp(mx);
+  std::vector<std::complex<R>> bi(mx), bo(mx), tmp(mx);
   for (int y = 0; y < ny; ++y) fx.run(a + (long long)y * nx, 1, sign, bi.data(), bo.data(), tmp.data());
   for (int x = 0; x < nx; ++x) fy.run(a + x, nx, sign, bi.data(), bo.data(), tmp.data());
 }
@@ -715,17 +724,19 @@ void potential_fine(const Grid& g, double zs, double ze, long long i0, long long
 // or a given complex transmission when tc != nullptr.  If ckpt != nullptr,
 // the wave entering slice k (real space) is stored in ckpt[k].
 // ---------------------------------------------------------------------------
-void multislice(const Grid& g, const cd* P, int S, const double* V, const cd* tc, cd* psi,
-                cd* ckpt) {
+template <class R>
+void multislice_t(const Grid& g, const std::complex<R>* P, int S, const R* V, const std::complex<R>* tc,
+                  std::complex<R>* psi, std::complex<R>* ckpt) {
+  typedef std::complex<R> C;
   const long long N = g.n();
   for (int k = 0; k < S; ++k) {
     if (ckpt) std::copy(psi, psi + N, ckpt + (long long)k * N);
     for (long long i = 0; i < N; ++i) {
-      cd t;
+      C t;
       if (tc) t = tc[(long long)k * N + i];
       else {
-        const double v = V[(long long)k * N + i];
-        t = cd(std::cos(v), std::sin(v));
+        const R v = V[(long long)k * N + i];
+        t = C(std::cos(v), std::sin(v));
       }
       psi[i] *= t;
     }
@@ -735,14 +746,18 @@ void multislice(const Grid& g, const cd* P, int S, const double* V, const cd* tc
     fft2(psi, g.nx, g.ny, +1);
   }
 }
+void multislice(const Grid& g, const cd* P, int S, const double* V, const cd* tc, cd* psi, cd* ckpt) {
+  multislice_t<double>(g, P, S, V, tc, psi, ckpt);
+}
 
 // sum_square_over_Det (device_kernels.cuh:534-579): closed band g2_min <= g^2 <= g2_max.
-double detector_sum(const Grid& g, const cd* Psi, double g2min, double g2max) {
+template <class R>
+double detector_sum(const Grid& g, const std::complex<R>* Psi, double g2min, double g2max) {
   double s = 0.0;
   for (int iy = 0; iy < g.ny; ++iy)
     for (int ix = 0; ix < g.nx; ++ix) {
       const double g2 = g.gx(ix) * g.gx(ix) + g.gy(iy) * g.gy(iy);
-      if (g2min <= g2 && g2 <= g2max) s += std::norm(Psi[(long long)iy * g.nx + ix]);
+      if (g2min <= g2 && g2 <= g2max) s += (double)std::norm(Psi[(long long)iy * g.nx + ix]);
     }
   return s;
 }
@@ -1158,6 +1173,52 @@ int orc_stem(int nx, int ny, double lx, double ly, double lambda, const double* 
   for (int b = 0; b < B; ++b) {
     probe(g, l, xb[b], yb[b], psi.data());
     multislice(g, Pc, S, V, nullptr, psi.data(), nullptr);
+    for (int d = 0; d < ndet; ++d) out[(long long)d * B + b] = w * detector_sum(g, psi.data(), g2min[d], g2max[d]);
+  }
+#endif
+  return used;
+}
+
+// The same STEM loop with the wave, transmission, propagator and transforms
+// in FP32 (probe built in FP64 and rounded; detector sums accumulated in
+// FP64): the FP32 variant of the CPU baseline, timed beside the FP64 oracle.
+int orc_stem_f32(int nx, int ny, double lx, double ly, double lambda, const double* lens,
+                 double defocus_shift, const double* P, int S, const double* V, int B,
+                 const double* xb, const double* yb, int ndet, const double* g2min,
+                 const double* g2max, double w, double* out, int nthreads) {
+  typedef std::complex<float> cf;
+  Grid g(nx, ny, lx, ly, 0.0, 0);
+  Lens l(lens, lambda, defocus_shift);
+  const long long N = g.n();
+  std::vector<cf> Pf(N);
+  std::vector<float> Vf((size_t)S * N);
+  for (long long i = 0; i < N; ++i) Pf[i] = cf((float)P[2 * i], (float)P[2 * i + 1]);
+  for (long long i = 0; i < (long long)S * N; ++i) Vf[i] = (float)V[i];
+  int used = 1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+  {
+#pragma omp single
+    used = omp_get_num_threads();
+    std::vector<cd> p64(N);
+    std::vector<cf> psi(N);
+#pragma omp for schedule(dynamic, 1)
+    for (int b = 0; b < B; ++b) {
+      probe(g, l, xb[b], yb[b], p64.data());
+      for (long long i = 0; i < N; ++i) psi[i] = cf((float)p64[i].real(), (float)p64[i].imag());
+      multislice_t<float>(g, Pf.data(), S, Vf.data(), nullptr, psi.data(), nullptr);
+      for (int d = 0; d < ndet; ++d) out[(long long)d * B + b] = w * detector_sum(g, psi.data(), g2min[d], g2max[d]);
+    }
+  }
+#else
+  (void)nthreads;
+  std::vector<cd> p64(N);
+  std::vector<cf> psi(N);
+  for (int b = 0; b < B; ++b) {
+    probe(g, l, xb[b], yb[b], p64.data());
+    for (long long i = 0; i < N; ++i) psi[i] = cf((float)p64[i].real(), (float)p64[i].imag());
+    multislice_t<float>(g, Pf.data(), S, Vf.data(), nullptr, psi.data(), nullptr);
     for (int d = 0; d < ndet; ++d) out[(long long)d * B + b] = w * detector_sum(g, psi.data(), g2min[d], g2max[d]);
   }
 #endif

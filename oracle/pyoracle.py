@@ -69,6 +69,8 @@ def lib():
         L.orc_stem.argtypes = [_I, _I, _D, _D, _D, _PD, _D, _PD, _I, _PD, _I, _PD, _PD,
                                _I, _PD, _PD, _D, _PD, _I]
         L.orc_stem.restype = _I
+        L.orc_stem_f32.argtypes = L.orc_stem.argtypes
+        L.orc_stem_f32.restype = _I
         L.orc_adjoint.argtypes = [_I, _I, _PD, _I, _PD, _I, _PD, _PD, _PD, _PD]
         L.orc_gauss_hermite.argtypes = [_I, _PD, _PD]
         L.orc_pctf_image.argtypes = [_I, _I, _D, _D, _D, _PD, _D, _D, _I, _PD, _PD, _PD, _PD]
@@ -396,13 +398,17 @@ def thickness(P, V, psi, lx, ly, g2min, g2max, stride=1):
 
 
 def stem(nx, ny, lx, ly, lam, lens, defocus_shift, P, V, xb, yb, g2min, g2max, w=1.0,
-         nthreads=0):
+         nthreads=0, fp32=False):
+    """Detector sums [ndet][B] of probes at (xb, yb), one OpenMP thread per
+    probe.  fp32: the same loop with the wave, tables and transforms in FP32
+    (a timing variant; the oracle is the FP64 loop)."""
     la = lens_array(lens)
     xb, yb = _f64(xb), _f64(yb)
     g2min, g2max = _f64(g2min), _f64(g2max)
     B = len(xb)
     out = np.zeros((len(g2min), B))
-    used = lib().orc_stem(nx, ny, lx, ly, lam, _pd(la), defocus_shift,
+    fn = lib().orc_stem_f32 if fp32 else lib().orc_stem
+    used = fn(nx, ny, lx, ly, lam, _pd(la), defocus_shift,
                           _c128(P).view(np.float64).ctypes.data_as(_PD), V.shape[0],
                           _pd(_f64(V)), B, _pd(xb), _pd(yb), len(g2min), _pd(g2min),
                           _pd(g2max), w, _pd(out), int(nthreads))

@@ -270,6 +270,24 @@ def test_vacuum_probe_detector_counts():
         assert abs(s - count * n ** 4) < 1e-8 * max(count * n ** 4, 1)
 
 
+def test_fp32_stem_variant_tracks_the_fp64_oracle():
+    """The FP32 timing variant of the STEM loop agrees with the FP64 loop to
+    FP32 accuracy, and differs from it (so it really runs in FP32)."""
+    rng = np.random.default_rng(11)
+    n, L, S = 64, 16.0, 6
+    lam = O.constants(300.0)[0]
+    V = _smooth_phase(rng, S, n, 0.8)
+    P = O.propagator(n, n, L, L, 2.0, True, lam)
+    lens = {"outer_aper_mrad": 25.0}
+    xb, yb = [3.0, 8.5, 12.0], [4.0, 9.5, 2.5]
+    g2min = np.array([(np.sin(20e-3) / lam) ** 2, 0.0])
+    g2max = np.array([(np.sin(60e-3) / lam) ** 2, (np.sin(20e-3) / lam) ** 2])
+    r64, _ = O.stem(n, n, L, L, lam, lens, 0.0, P, V, xb, yb, g2min, g2max, nthreads=2)
+    r32, _ = O.stem(n, n, L, L, lam, lens, 0.0, P, V, xb, yb, g2min, g2max, nthreads=2, fp32=True)
+    e = np.linalg.norm(r32 - r64) / np.linalg.norm(r64)
+    assert 1e-12 < e < 1e-4
+
+
 def test_weak_phase_object_is_linear():
     rng = np.random.default_rng(4)
     n = 32
