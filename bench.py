@@ -13,6 +13,7 @@ Prints one JSON line.  --gpus N > 1 is refused: this build measures one GPU.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -398,6 +399,11 @@ def main():
         # whole slice step against one read + write of the wave per probe-slice
         # (16 B per pixel) plus the shared potential and propagator tables
         "slice_roofline_frac": ps * (16 * N + 12 * N / G) / HBM_PEAK,
+        # arithmetic, reported only (HBM bounds the path): two 2D FFT pairs'
+        # worth of radix-2 flops per probe-slice, 2 * 5 N log2 N
+        "flops": {"per_probe_slice": 10.0 * N * math.log2(N),
+                  "achieved_TFLOPs": ps * 10.0 * N * math.log2(N) / 1e12,
+                  "fp32_vector_peak_TFLOPs": 157.3},
         "band_skip": {"threshold": 0.0 if args.no_band_skip else 1e-9, "skipped_columns": band[1] - band[0] + 1 if band[1] >= band[0] else 0,
                       "of": N_GRID},
         "batched_vs_plain": {"ratio": ps / plain_ps, "plain_probe_slices_per_s": plain_ps,
