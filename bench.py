@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--only-stem", action="store_true",
                     help="time the batched STEM steps only (profiling runs): no pass timing, plain path, "
                          "adjoint or CPU baseline")
+    ap.add_argument("--only-ragged", action="store_true",
+                    help="run only the ragged mix, three times (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device 5 times (known bytes for HBM counter scaling)")
     return ap.parse_args()
@@ -277,6 +279,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "device copy", "bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
                           "launches": 5}))
+        return
+    if args.only_ragged:
+        for _ in range(3):
+            rag_ps, rag_work = ragged_mix(em, torch)
+            print(json.dumps({"metric": "probe-slices/s (ragged mix only)", "value": rag_ps,
+                              "probe_slices": rag_work}), flush=True)
         return
     at0 = em.specimen.srtio3(*CELLS, a=A_SRTIO3)
     L = at0["lx"]
