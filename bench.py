@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--only-stem", action="store_true",
                     help="time the batched STEM steps only (profiling runs): no pass timing, plain path, "
                          "adjoint or CPU baseline")
+    ap.add_argument("--grid", type=int, default=N_GRID,
+                    help="grid size of the batched STEM steps; other than 1024 only with --only-stem")
     ap.add_argument("--only-ragged", action="store_true",
                     help="run only the ragged mix, three times (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -262,7 +264,13 @@ def ragged_mix(em, torch, n_req=8, seed=7):
 
 
 def main():
+    global N_GRID
     args = parse()
+    if args.grid != N_GRID and not args.only_stem:
+        print(json.dumps({"error": "--grid other than 1024 needs --only-stem (the roofline's traffic "
+                                   "constants are measured at 1024^2)"}))
+        sys.exit(2)
+    N_GRID = args.grid
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
                           "n_gpus_requested": args.gpus}))
@@ -330,7 +338,8 @@ def main():
 
     if args.only_stem:
         print(json.dumps({"metric": "probe-slices/s (batched STEM steps only)", "value": ps, "steps": args.steps,
-                          "probes_per_step": npos, "tuning": plan.tuning()}), flush=True)
+                          "grid": N_GRID, "probes_per_step": npos, "probe_group": plan.group_size(args.batch),
+                          "tuning": plan.tuning()}), flush=True)
         return
 
     # Dominant kernels at the real shape: one batch of the same STEM slice
