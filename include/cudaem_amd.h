@@ -102,9 +102,9 @@ em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* 
  * row / column pass (the probes each workgroup processes follow from it);
  * col_tile: columns per column-pass workgroup (0 = default, or the
  * alternative tile of the grid size); probe_group: probes carried together
- * through all slices (0 = about 320 MiB of waves, evened out over a batch:
- * with band skipping the passes touch about 3/4 of it, near the 256 MiB
- * last-level cache); col_variant: exchange scheme of the per-slice
+ * through all slices (0 = about 320 MiB of waves, at least 64 waves of more
+ * than 16 MiB, evened out over a batch: with band skipping the passes touch
+ * about 3/4 of it, near the 256 MiB last-level cache); col_variant: exchange scheme of the per-slice
  * column pass (0 = automatic, 1 = workgroup barrier per exchange,
  * 2 = one wavefront per column after the first stage); row_variant: row
  * pass of the STEM slice loop (0 = precomputed transmission held in
