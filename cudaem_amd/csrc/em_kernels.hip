@@ -7,7 +7,10 @@
 // so one slice is one row pass plus one column pass, each reading and writing
 // the wave once.  Between passes the wave is kept in (y, kx) layout, row-major.
 // Both passes reuse the transmission / propagator values held in registers
-// for every probe a workgroup processes.
+// for every probe a workgroup processes (the column pass reads P at the
+// multiply instead where holding it would spill: 768, 2048).  Chunk-table
+// modes give each workgroup row its own wave and table (ragged batches, a
+// potential stack per wave).
 #include <hip/hip_ext.h>
 
 #include <cstdint>
@@ -307,8 +310,6 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
   return __dadd_rn(__dmul_rn(gx, gx), __dmul_rn(gy, gy));
 }
 
-// At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
-// 8-column tile then share a CU, one loading while the other computes.
 // Whether the column pass reads P right before the multiply instead of
 // holding it in registers through the forward FFT.  Held, it makes the 768
 // and 2048 kernels spill at 4 waves per SIMD (16 B and 148 B of scratch);
@@ -320,6 +321,8 @@ struct ColLateP {
   static constexpr bool value = NY == 768 || NY == 2048;
 };
 
+// At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
+// 8-column tile then share a CU, one loading while the other computes.
 template <int NY, int COLS_, unsigned MODE>
 __global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 void col_pass_kernel(ColArgs a) {
