@@ -91,8 +91,9 @@ em_status em_plan_set_probe_group(em_plan plan, int probes_per_group);
 em_status em_plan_group_size(em_plan plan, int nprobe, int* probes_per_group);
 /* Per-launch timing of em_stem's slice loop (no reference counterpart: it
  * replaces reading kernel durations off a profiler).  While on, em_stem runs
- * its batches without graph capture and brackets every interior row pass and
- * column pass with a pair of HIP events; em_plan_pass_times then returns the
+ * its batches without graph capture, and every interior row pass and column
+ * pass launch records its own start and end in a pair of HIP events
+ * (hipExtLaunchKernel); em_plan_pass_times then returns the
  * average launch duration of each (ms) and the number of launches averaged,
  * and clears the records. */
 em_status em_plan_set_pass_timing(em_plan plan, int on);
