@@ -714,6 +714,48 @@ def test_benchmark_size_properties(em):
     assert rel_l2(stem, det) < 1e-5
 
 
+def test_runs_are_bitwise_reproducible(em):
+    """No atomics, ordered reductions: repeated batched STEM, ragged and
+    adjoint runs on the same inputs give bitwise identical results."""
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 4)
+    n = 128
+    xs, ys = sp.raster(6, 5, at["lx"], at["ly"])
+    lens = probe_lens(20.0)
+    with em.Plan(n, n, at["lx"], at["ly"], 1.9525, 300.0, max_batch=8) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+        a = _np(p.stem(lens, xs, ys, [40.0, 10.0], [120.0, 40.0]))
+        b = _np(p.stem(lens, xs, ys, [40.0, 10.0], [120.0, 40.0]))
+        assert np.array_equal(a, b)
+        V = p.potential_slices()
+        S = V.shape[0]
+        psi0 = p.build_probes(lens, xs[:6], ys[:6])
+        grads = []
+        for _ in range(2):
+            psi = psi0.clone()
+            ck = torch.empty((S, 6, n, n), dtype=torch.complex64, device="cuda")
+            p.multislice_forward(psi, V=V, ckpt=ck)
+            g = (psi * 1e-4).contiguous()
+            grads.append(_np(p.multislice_adjoint(ck, g, V=V)))
+        assert np.array_equal(grads[0], grads[1])
+    outs = []
+    for _ in range(2):
+        plans, Vs, psis = [], [], []
+        for nn, e0, B in ((128, 300.0, 2), (128, 200.0, 3), (64, 300.0, 2)):
+            q = em.Plan(nn, nn, at["lx"], at["ly"], 1.9525, e0, max_batch=B)
+            q.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+            Vs.append(q.potential_slices())
+            psis.append(q.build_probes(lens, xs[:B], ys[:B]))
+            plans.append(q)
+        outs.append([_np(o) for o in em.ragged(plans, Vs, psis, [10.0], [40.0], stride=3)])
+        for q in plans:
+            q.close()
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+    torch.cuda.synchronize()
+
+
 def test_band_skip_is_below_fp32_resolution(em):
     """Skipping the propagator columns below 1e-9 (relative) changes the
     exit waves and the adjoint gradient by less than FP32 resolution; it skips
