@@ -53,6 +53,26 @@ struct Graph {
   }
 };
 
+// Events owned by one call: destroyed on every return path.
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  EventSet() = default;
+  EventSet(const EventSet&) = delete;
+  EventSet& operator=(const EventSet&) = delete;
+  ~EventSet() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  hipError_t make(hipEvent_t* out, unsigned flags = hipEventDefault) {
+    hipEvent_t e = nullptr;
+    const hipError_t r = hipEventCreateWithFlags(&e, flags);
+    if (r == hipSuccess) {
+      ev.push_back(e);
+      *out = e;
+    }
+    return r;
+  }
+};
+
 }  // namespace
 
 struct em_plan_s {
@@ -1230,9 +1250,10 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
   EM_HIP(p, p->ex.ensure(sizeof(float2) * B * p->grid.nx));
   EM_HIP(p, p->ey.ensure(sizeof(float2) * B * p->grid.ny));
   // order the private graph stream after the caller's stream
-  hipEvent_t ev0, ev1;
-  EM_HIP(p, hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-  EM_HIP(p, hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  EventSet evs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  EM_HIP(p, evs.make(&ev0, hipEventDisableTiming));
+  EM_HIP(p, evs.make(&ev1, hipEventDisableTiming));
   EM_HIP(p, hipEventRecord(ev0, p->stream));
   EM_HIP(p, hipStreamWaitEvent(p->gstream, ev0, 0));
   const hipStream_t user = p->stream;
@@ -1283,8 +1304,6 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
   }
   (void)hipEventRecord(ev1, p->gstream);
   (void)hipStreamWaitEvent(user, ev1, 0);
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   return st;
 }
 
@@ -1470,17 +1489,18 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   EM_HIP(p0, hipMemcpy(p0->chunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
   const PassChunk* dchunks = p0->chunks.as<PassChunk>();
 
-  hipEvent_t fork, join;
-  std::vector<hipEvent_t> done(cls.size());
-  EM_HIP(p0, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-  EM_HIP(p0, hipEventCreateWithFlags(&join, hipEventDisableTiming));
-  for (auto& e : done) EM_HIP(p0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  EventSet evs;
+  hipEvent_t fork = nullptr, join = nullptr;
+  std::vector<hipEvent_t> done(cls.size(), nullptr);
+  EM_HIP(p0, evs.make(&fork, hipEventDisableTiming));
+  EM_HIP(p0, evs.make(&join, hipEventDisableTiming));
+  for (auto& e : done) EM_HIP(p0, evs.make(&e, hipEventDisableTiming));
   EM_HIP(p0, hipEventRecord(fork, user));
   EM_HIP(p0, hipStreamWaitEvent(p0->gstream, fork, 0));
   Graph graph;
   em_status st = p0->hip(hipStreamBeginCapture(p0->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
   hipEvent_t cap_fork = nullptr;
-  if (st == EM_OK) st = p0->hip(hipEventCreateWithFlags(&cap_fork, hipEventDisableTiming), "event");
+  if (st == EM_OK) st = p0->hip(evs.make(&cap_fork, hipEventDisableTiming), "event");
   if (st == EM_OK) st = p0->hip(hipEventRecord(cap_fork, p0->gstream), "record");
   for (size_t c = 0; c < cls.size() && st == EM_OK; ++c) {
     // each class is a branch of the graph on its first plan's private stream
@@ -1535,10 +1555,6 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   if (st == EM_OK) st = p0->hip(hipEventRecord(join, p0->gstream), "record");
   if (st == EM_OK) st = p0->hip(hipStreamWaitEvent(user, join, 0), "join");
   if (st == EM_OK) st = p0->hip(hipStreamSynchronize(p0->gstream), "sync");
-  (void)hipEventDestroy(fork);
-  (void)hipEventDestroy(join);
-  if (cap_fork) (void)hipEventDestroy(cap_fork);
-  for (auto& e : done) (void)hipEventDestroy(e);
   return st;
 }
 
@@ -1548,8 +1564,9 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
                                     float* col_ms) {
   if (!p || !V || !wave || nprobe <= 0 || iters <= 0 || !row_ms || !col_ms) return EM_ERR_INVALID;
   float2* w = reinterpret_cast<float2*>(wave);
-  hipEvent_t e[3];
-  for (auto& x : e) EM_HIP(p, hipEventCreate(&x));
+  EventSet evs;
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  for (auto& x : e) EM_HIP(p, evs.make(&x));
   // the row pass as em_stem runs it: transmission precomputed per slice
   EM_HIP(p, p->tmp.ensure(sizeof(float2) * p->N()));
   EM_HIP(p, launch_transmission(V, p->tmp.as<float2>(), p->N(), p->stream));
@@ -1575,7 +1592,6 @@ extern "C" em_status em_time_passes(em_plan p, const float* V, void* wave, int n
   float a = 0, b = 0;
   if (st == EM_OK) st = p->hip(hipEventElapsedTime(&a, e[0], e[1]), "elapsed");
   if (st == EM_OK) st = p->hip(hipEventElapsedTime(&b, e[1], e[2]), "elapsed");
-  for (auto& x : e) (void)hipEventDestroy(x);
   *row_ms = a / iters;
   *col_ms = b / iters;
   return st;
@@ -1672,11 +1688,12 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   }
   // potential slices: the real parts of the unit phasors (any values do)
   float* Vt = reinterpret_cast<float*>(T);
-  hipEvent_t e0, e1;
+  EventSet evs;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   em_status st = p->hip(launch_phase_fill(T, N * nslices, p->stream), "phase fill");
   if (st == EM_OK) st = p->hip(launch_phase_fill(w, N * nprobe, p->stream), "phase fill");
-  if (st == EM_OK) st = p->hip(hipEventCreate(&e0), "event");
-  if (st == EM_OK) st = p->hip(hipEventCreate(&e1), "event");
+  if (st == EM_OK) st = p->hip(evs.make(&e0), "event");
+  if (st == EM_OK) st = p->hip(evs.make(&e1), "event");
   // best of three timed runs after a warm-up run: the minimum is the most
   // repeatable statistic when other work on the device adds noise
   auto measure = [&](double* ms) -> em_status {
@@ -1761,8 +1778,6 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
     p->tune = (st != EM_OK || b <= a) ? chosen : start;
     cur = std::min(a, b);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(p->stream);
   (void)hipFree(T);
   (void)hipFree(w);
