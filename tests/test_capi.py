@@ -69,6 +69,23 @@ def test_validation_before_device_work(C):
     assert L.em_multislice_ragged(0, None, None, None, None, None, 0, None, None, 1, None, None) == C.EM_ERR_INVALID
 
 
+def test_every_plan_entry_point_rejects_a_null_plan(C):
+    """Each exported function that takes a plan (or plans) returns
+    EM_ERR_INVALID for null handles and zero sizes, before any device call."""
+    L = C.lib()
+    checked = 0
+    for name in sorted(C.EXPORTS):
+        if name.startswith("em_host") or name in ("em_fft_size_supported", "em_plan_last_error"):
+            continue
+        f = getattr(L, name)
+        args = [0 if t in (ctypes.c_int, ctypes.c_longlong) else 0.0 if t in (ctypes.c_double, ctypes.c_float)
+                else None for t in f.argtypes]
+        assert f(*args) == C.EM_ERR_INVALID, name
+        checked += 1
+    assert checked >= 35
+    assert L.em_plan_last_error(None) == b"null plan"
+
+
 def test_host_constants_match_oracle(C):
     for kv in (80.0, 200.0, 300.0):
         l, g, s = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
