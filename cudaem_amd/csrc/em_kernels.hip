@@ -134,7 +134,14 @@ struct ColCfg {
   static constexpr int T = FftCfg<N>::T;
   static constexpr int COLS = COLS_;
   static constexpr int THREADS = COLS * T;
-  static constexpr int STRIDE = LineLds<N>::size + 1;  // odd stride: columns on distinct banks
+  // Column buffers 16/COLS eight-byte banks apart.  A wavefront holds
+  // 16/COLS consecutive tau of every column per 16 lanes, and consecutive
+  // tau of one column sit one bank apart in every exchange of the line
+  // layout: with this offset the 16 lanes of an LDS cycle hit 16 distinct
+  // banks (an offset of 1 put columns c and c + 1 of neighbouring tau on one
+  // bank: 2-way conflicts for 8-column tiles, 4-way for 4-column tiles).
+  static_assert(COLS >= 1 && 16 % COLS == 0, "column tile must divide 16");
+  static constexpr int STRIDE = LineLds<N>::size + 16 / COLS;
   // Twiddles from an LDS table after the column buffers where the register
   // form would read the global table at every use (768); elsewhere they are
   // formed in registers.
