@@ -1719,7 +1719,20 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   if (st == EM_OK) st = measure(&cur);
   // coordinate descent over the launch-shape parameters, two sweeps
   const int groups[] = {0, 8, 16, 24, 32, 40, 48, 64, nprobe};
-  const int rowb[] = {512, 1024, 2048, 4096, 8192, 16384};
+  // row-pass targets: powers of two, and whole rounds of the workgroups the
+  // chip holds at once (1024^2: 256 CUs x 3 = 768 ran 3% faster per
+  // probe-slice than 2048, whose last round is partly empty)
+  std::vector<int> rowb = {512, 1024, 2048, 4096, 8192, 16384};
+  {
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        row_blocks_per_cu(p->grid.nx, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), &occ) == hipSuccess &&
+        occ > 0 && cus > 0)
+      for (int k : {1, 2, 4}) rowb.push_back(k * occ * cus);
+    std::sort(rowb.begin(), rowb.end());
+    rowb.erase(std::unique(rowb.begin(), rowb.end()), rowb.end());
+  }
   const int colb[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
@@ -1727,12 +1740,12 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
     for (int par = 0; par < 6 && st == EM_OK; ++par) {
       const int* cand = par == 0   ? groups
-                        : par == 1 ? rowb
+                        : par == 1 ? rowb.data()
                         : par == 2 ? colb
                         : par == 3 ? tiles
                         : par == 4 ? variants
                                    : row_variants;
-      const int nc = par == 0 ? 9 : par == 1 ? 6 : par == 2 ? 7 : par == 5 ? 3 : 2;
+      const int nc = par == 0 ? 9 : par == 1 ? (int)rowb.size() : par == 2 ? 7 : par == 5 ? 3 : 2;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;

@@ -628,12 +628,19 @@ bool fft_size_supported(int n) {
   }
 }
 
+// dynamic LDS of a row-pass workgroup: its line buffers and twiddle table
+template <int NX>
+constexpr size_t row_lds() {
+  constexpr int LINES = kRowThreads / FftCfg<NX>::T;
+  constexpr int TAB = FftCfg<NX>::TwTab::ok ? FftCfg<NX>::TwTab::entries : 0;
+  return ((size_t)LINES * LineLds<NX>::size + TAB) * sizeof(float2);
+}
+
 template <int NX>
 static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   constexpr int LINES = kRowThreads / FftCfg<NX>::T;
   const dim3 grid(a.ny / LINES, (a.nprobe + a.pb - 1) / a.pb);
-  constexpr int TAB = FftCfg<NX>::TwTab::ok ? FftCfg<NX>::TwTab::entries : 0;
-  const size_t lds = ((size_t)LINES * LineLds<NX>::size + TAB) * sizeof(float2);
+  constexpr size_t lds = row_lds<NX>();
   switch (mode) {
 #define EM_ROW_CASE(M)                                                            \
   case (M):                                                                       \
@@ -767,6 +774,32 @@ static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
     return col_launch_one<NY, DefaultCols<NY>::value, (M)>(a, s);
     EM_COL_MODES(EM_COL_CASE)
 #undef EM_COL_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <int NX>
+static hipError_t row_occupancy(unsigned mode, int* n) {
+  switch (mode) {
+#define EM_ROW_OCC(M) \
+  case (M):           \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, row_pass_kernel<NX, (M)>, kRowThreads, row_lds<NX>());
+    EM_ROW_MODES(EM_ROW_OCC)
+#undef EM_ROW_OCC
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t row_blocks_per_cu(int nx, unsigned mode, int* n) {
+  if (!n) return hipErrorInvalidValue;
+  switch (nx) {
+#define EM_CASE(N) \
+  case N:          \
+    return row_occupancy<N>(mode, n);
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
     default:
       return hipErrorInvalidValue;
   }

@@ -132,6 +132,9 @@ enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };
 
 hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
 hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
+// Row-pass workgroups of this mode and row length that one CU holds at once
+// (the occupancy limit of its registers and LDS).
+hipError_t row_blocks_per_cu(int nx, unsigned mode, int* n);
 // Arms a pair of timing events for the next row or column pass launched from
 // the calling thread: that launch records the kernel's own start and end in
 // them (hipExtLaunchKernel), without the dispatch gap that events queued
