@@ -35,10 +35,10 @@ U_RMS = 0.08
 SEED = 1234
 HBM_PEAK = 8.0e12
 # HBM traffic per probe of one launch of the fused passes at 1024^2, groups of
-# 40 probes, band skip 1e-9, row_blocks 4096 / col_blocks 8192: rocprofv3 --pmc
-# FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000), factors from a 1 GiB device
-# copy; profiles/batched_stem_1024_hbm_traffic.md
-PMC_BYTES_PER_PROBE = {"column pass": (281.4 + 247.9) / 40 * 2**20, "row pass": (301.1 + 241.0) / 40 * 2**20}
+# 37 probes, band skip 1e-9, row_blocks 768 / col_blocks 4096 (the autotuner's
+# usual choice): rocprofv3 --pmc FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000),
+# factors from a 1 GiB device copy; profiles/batched_stem_1024_hbm_traffic.md
+PMC_BYTES_PER_PROBE = {"column pass": (265.74 + 229.18) / 37 * 2**20, "row pass": (230.21 + 222.92) / 37 * 2**20}
 
 
 def parse():
@@ -52,6 +52,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--npos", type=int, default=0, help="scan positions per step (0 = the full 64x64 raster)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--row-blocks", type=int, default=0,
+                    help="row-pass workgroup target instead of the default (with --no-autotune)")
+    ap.add_argument("--col-blocks", type=int, default=0,
+                    help="column-pass workgroup target instead of the default (with --no-autotune)")
     ap.add_argument("--plain-probes", type=int, default=4)
     ap.add_argument("--no-band-skip", action="store_true",
                     help="compute every propagator column (no band-limit skipping)")
@@ -308,7 +312,12 @@ def main():
     plan.set_probe_group(args.group)
     band = plan.set_band_skip(0.0 if args.no_band_skip else 1e-9)
     tune_ms = None
-    if not args.no_autotune:
+    if args.no_autotune:
+        if args.row_blocks:
+            plan.set_tuning(row_blocks=args.row_blocks)
+        if args.col_blocks:
+            plan.set_tuning(col_blocks=args.col_blocks)
+    else:
         tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=8)
         if args.group:
             plan.set_tuning(probe_group=args.group)
