@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -360,6 +361,28 @@ em_status forward_group(em_plan p, const float* V, const float2* t, int S, float
   return EM_OK;
 }
 
+// Row-pass workgroups the device holds at once (occupancy x CUs), 0 if the
+// query fails.
+int row_slots(em_plan p) {
+  int dev = 0, cus = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      row_blocks_per_cu(p->grid.nx, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), &occ) != hipSuccess)
+    return 0;
+  return std::max(0, occ) * std::max(0, cus);
+}
+
+// Default row-pass target: the fewest probe chunks whose workgroups fill whole
+// rounds of the resident slots.  1024^2 on 256 CUs: 3 chunks of 256
+// workgroups, one round of 768 (5% faster per probe-slice than 4096, whose
+// last round is partly empty); 2048^2: 3 chunks of 1024, four rounds.
+int whole_round_row_target(em_plan p) {
+  const int slots = row_slots(p);
+  const int per_chunk = p->grid.ny / row_lines_per_block(p->grid.nx);
+  if (slots <= 0 || per_chunk <= 0) return 4096;
+  return per_chunk * (slots / std::gcd(per_chunk, slots));
+}
+
 int group_size(em_plan p, int B) {
   int g = p->tune.probe_group;
   if (g <= 0) {
@@ -415,6 +438,7 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
   }
   EM_HIP(raw, hipDeviceSynchronize());
   compute_band_skip(raw);
+  p->tune.row_blocks = whole_round_row_target(raw);
   *out = p.release();
   return EM_OK;
 }
@@ -1722,17 +1746,11 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   // row-pass targets: powers of two, and whole rounds of the workgroups the
   // chip holds at once (1024^2: 256 CUs x 3 = 768 ran 3% faster per
   // probe-slice than 2048, whose last round is partly empty)
-  std::vector<int> rowb = {512, 1024, 2048, 4096, 8192, 16384};
-  {
-    int dev = 0, cus = 0, occ = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        row_blocks_per_cu(p->grid.nx, row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT), &occ) == hipSuccess &&
-        occ > 0 && cus > 0)
-      for (int k : {1, 2, 4}) rowb.push_back(k * occ * cus);
-    std::sort(rowb.begin(), rowb.end());
-    rowb.erase(std::unique(rowb.begin(), rowb.end()), rowb.end());
-  }
+  std::vector<int> rowb = {512, 1024, 2048, 4096, 8192, 16384, whole_round_row_target(p)};
+  if (const int slots = row_slots(p))
+    for (int k : {1, 2, 4}) rowb.push_back(k * slots);
+  std::sort(rowb.begin(), rowb.end());
+  rowb.erase(std::unique(rowb.begin(), rowb.end()), rowb.end());
   const int colb[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};

@@ -100,7 +100,9 @@ em_status em_plan_set_pass_timing(em_plan plan, int on);
 em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* n_row, int* n_col);
 /* Launch shape of the fused passes (no reference counterpart: the reference
  * relies on cuFFT's planner).  row_blocks / col_blocks: target workgroups of a
- * row / column pass (the probes each workgroup processes follow from it);
+ * row / column pass (the probes each workgroup processes follow from it; the
+ * row default is the fewest probe chunks whose workgroups fill whole rounds of
+ * the row workgroups the device holds at once, e.g. 768 at 1024^2 on 256 CUs);
  * col_tile: columns per column-pass workgroup (0 = default, or the
  * alternative tile of the grid size); probe_group: probes carried together
  * through all slices (0 = about 320 MiB of waves, at least 64 waves of more

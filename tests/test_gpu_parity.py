@@ -673,6 +673,16 @@ def test_tuning_variants_agree(em):
     assert rel_l2(touts[1], touts[0]) < 1e-6
 
 
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 2048])
+def test_default_row_target_is_whole_probe_chunks(em, n):
+    """The default row-pass target is a positive whole number of probe chunks
+    (ny / rows per workgroup workgroups each)."""
+    lines = 256 // {64: 8, 128: 16, 256: 16, 512: 64, 768: 64, 1024: 64, 2048: 128}[n]
+    with em.Plan(n, n, 20.0, 20.0, 2.0, 300.0, max_batch=2) as p:
+        rb = p.tuning()["row_blocks"]
+    assert rb > 0 and rb % (n // lines) == 0, rb
+
+
 def test_benchmark_size_properties(em):
     """The benchmark's own configuration (SrTiO3 10 x 10 x 40, 1024^2, 80
     slices, 300 kV, 25 mrad), checked through properties that hold at any
