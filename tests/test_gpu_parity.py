@@ -439,14 +439,17 @@ def test_adjoint_matches_oracle(em):
     assert e < TOL_INT
 
 
-# (grid, kV, probes, cells across, cells thick): one request per grid size,
-# and a mix where requests of one size share every fused pass of a slice step
-# although their slice counts, voltages (potential scaling, propagator),
-# probe counts and windows (band-limit skip range) differ
+# (grid n or (nx, ny), kV, probes, cells across, cells thick): one request per
+# grid size; a mix where requests of one size share every fused pass of a
+# slice step although their slice counts, voltages (potential scaling,
+# propagator), probe counts and windows (band-limit skip range) differ; and
+# non-square grids (a class per shape) beside a 512^2 request
 RAGGED_MIXES = {
     "distinct_sizes": [(128, 300.0, 2, 3, 3), (256, 200.0, 3, 3, 3), (64, 300.0, 1, 3, 3)],
     "shared_sizes": [(128, 300.0, 2, 3, 3), (128, 200.0, 3, 3, 5), (64, 300.0, 1, 3, 3),
                      (128, 300.0, 1, 3, 2), (128, 300.0, 2, 4, 4), (64, 200.0, 2, 3, 6)],
+    "non_square": [((128, 256), 300.0, 2, 3, 3), ((256, 128), 200.0, 2, 3, 4), ((128, 256), 300.0, 1, 3, 2),
+                   (512, 300.0, 1, 3, 3)],
 }
 
 
@@ -456,8 +459,9 @@ def test_ragged_matches_separate_runs(em, mix):
     import cudaem_amd.specimen as sp
     plans, Vs, psis, refs = [], [], [], []
     for n, e0, B, cells, nz in RAGGED_MIXES[mix]:
+        nx, ny = n if isinstance(n, tuple) else (n, n)
         at = sp.si_diamond(cells, cells, nz)
-        p = em.Plan(n, n, at["lx"], at["ly"], 1.35775, e0, max_batch=B)
+        p = em.Plan(nx, ny, at["lx"], at["ly"], 1.35775, e0, max_batch=B)
         p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
         V = p.potential_slices()
         xs = np.linspace(1, at["lx"] - 1, B)
