@@ -94,6 +94,9 @@ struct em_plan_s {
   int nR = 128;
   double r_min = 0.005, r_max = 3.0, tap = 0.85;
   std::map<int, ElementCoefs> custom;
+  // projected-potential splines by atomic number, for the current
+  // coefficients and spline parameters (a specimen change rebuilds none)
+  std::map<int, Spline> spline_cache;
   // fine slicing (projected_potential.cuh:73-91): a spline per atom entry of
   // every slice, from the element's 3D potential integrated over the slice
   struct Params3D {
@@ -545,6 +548,7 @@ em_status em_set_element_coefs(em_plan plan, int Z, int nk, const double* a, con
     k.d.assign(d, d + ng);
   }
   plan->custom[Z] = k;
+  plan->spline_cache.erase(Z);
   return EM_OK;
 }
 
@@ -555,6 +559,7 @@ em_status em_set_spline_params(em_plan plan, int nR, double r_min, double r_max,
   plan->r_min = r_min;
   plan->r_max = r_max;
   plan->tap = tap_frac;
+  plan->spline_cache.clear();
   return EM_OK;
 }
 
@@ -605,9 +610,13 @@ em_status em_plan_set_specimen(em_plan p, const em_atoms* at, int border, int* n
   if ((int)p->types.size() > 8) return p->fail(EM_ERR_UNSUPPORTED, "more than 8 elements");
   p->splines.clear();
   for (int Z : p->types) {
-    auto it = p->custom.find(Z);
-    const ElementCoefs k = it != p->custom.end() ? it->second : moliere_coefs(Z);
-    p->splines.push_back(build_spline(k, p->nR, p->r_min, p->r_max, p->tap));
+    auto c = p->spline_cache.find(Z);
+    if (c == p->spline_cache.end()) {
+      auto it = p->custom.find(Z);
+      const ElementCoefs k = it != p->custom.end() ? it->second : moliere_coefs(Z);
+      c = p->spline_cache.emplace(Z, build_spline(k, p->nR, p->r_min, p->r_max, p->tap)).first;
+    }
+    p->splines.push_back(c->second);
   }
   double rmax = 0;
   for (auto& s : p->splines) rmax = std::max(rmax, std::sqrt(s.rmax2));

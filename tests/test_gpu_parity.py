@@ -182,6 +182,42 @@ def test_spline_tables_match_oracle(em):
             np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
 
 
+def test_spline_cache_follows_coefficient_and_parameter_changes(em):
+    """A plan keeps its element splines across specimens.  Changing an
+    element's coefficients or the spline parameters must rebuild them: the
+    tables then equal those of a fresh plan set up the same way."""
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(2, 2, 2)
+    spec = (at["x"], at["y"], at["z"], at["Z"], at["occ"])
+    args = (64, 64, at["lx"], at["ly"], 1.9525, 300.0)
+    coefs = ((0.5, 1.0), (2.0, 10.0))
+    with em.Plan(*args) as p, em.Plan(*args) as q, em.Plan(*args) as r:
+        p.set_specimen(*spec)
+        _, before = p.spline_tables()
+        p.set_element_coefs(8, *coefs)
+        p.set_specimen(*spec)
+        Zs, after = p.spline_tables()
+        q.set_element_coefs(8, *coefs)
+        q.set_specimen(*spec)
+        _, fresh = q.spline_tables()
+        assert list(Zs) == [8, 22, 38]
+        assert not np.array_equal(before[1][0], after[1][0])  # oxygen rebuilt
+        for t in (1, 2):
+            assert np.array_equal(before[1][t], after[1][t])  # the others kept
+        for a, b in zip(after, fresh):
+            assert np.array_equal(a, b)
+        p.set_spline_params(nR=96)
+        p.set_specimen(*spec)
+        _, after = p.spline_tables()
+        r.set_element_coefs(8, *coefs)
+        r.set_spline_params(nR=96)
+        r.set_specimen(*spec)
+        _, fresh = r.spline_tables()
+        assert after[0].shape[1] == 96
+        for a, b in zip(after, fresh):
+            assert np.array_equal(a, b)
+
+
 def test_probe_matches_oracle(em):
     n, L = 256, 20.0
     lens = {"c_10": 35.0, "c_30": 1.2e4, "outer_aper_mrad": 21.0, "c_12": 8.0, "phi_12": 0.3}
