@@ -397,32 +397,54 @@ void run_STEM_simu(multislice_params<T>* mp, int /*GPU_ID*/, output_Multi_Data<T
     hip_ok(hipMemcpy(host.data(), out.p, sizeof(float) * host.size(), hipMemcpyDeviceToHost), "copy");
   } else {
     // general scanning: the atoms are re-cut around every probe and the
-    // probe is built at the window centre (multislice_params_dev.cuh:159)
-    Dev psi(N * 8), d(sizeof(float) * nd);
-    std::vector<float> hd(nd);
+    // probe is built at the window centre (multislice_params_dev.cuh:159).
+    // Each probe keeps its own potential stack and defocus; blocks of probes
+    // then run through their slices as the waves of one batch
+    // (em_multislice_forward_multi) instead of one wave per launch.
+    constexpr int kMaxProbeBlock = 16;
+    constexpr size_t kBlockBytes = size_t(8) << 30;
+    Dev psi(N * 8 * kMaxProbeBlock), d(sizeof(float) * nd * kMaxProbeBlock);
+    std::vector<float> hd((size_t)nd * kMaxProbeBlock);
     for (int iconf = 1; iconf <= nconf; ++iconf) {
       const Atom_Coord_Data<T> at = atoms_of_config(m, iconf);
-      for (int ip = 0; ip < npos; ++ip) {
-        const double xs = x[ip] - m.lx / 2, ys = y[ip] - m.ly / 2;
-        Atom_Coord_Data<T> cut;
-        for (size_t i = 0; i < at.size(); ++i)
-          if (at.x[i] >= xs - 2.0 && at.x[i] <= xs + m.lx + 2.0 && at.y[i] >= ys - 2.0 && at.y[i] <= ys + m.ly + 2.0)
-            cut.push_back(at.Z[i], (T)(at.x[i] - xs), (T)(at.y[i] - ys), at.z[i], at.occ.empty() ? T(1) : at.occ[i]);
-        if (cut.size() == 0) {
-          for (int k = 0; k < nd; ++k) host[(size_t)k * npos + ip] += w * (float)blank[k];
-          continue;
+      for (int ip = 0; ip < npos;) {
+        std::vector<std::unique_ptr<Dev>> V;
+        std::vector<const float*> Vp;
+        std::vector<int> Ss, pos;
+        size_t used = 0;
+        while (ip < npos && (int)pos.size() < kMaxProbeBlock && used < kBlockBytes) {
+          const double xs = x[ip] - m.lx / 2, ys = y[ip] - m.ly / 2;
+          Atom_Coord_Data<T> cut;
+          for (size_t i = 0; i < at.size(); ++i)
+            if (at.x[i] >= xs - 2.0 && at.x[i] <= xs + m.lx + 2.0 && at.y[i] >= ys - 2.0 &&
+                at.y[i] <= ys + m.ly + 2.0)
+              cut.push_back(at.Z[i], (T)(at.x[i] - xs), (T)(at.y[i] - ys), at.z[i], at.occ.empty() ? T(1) : at.occ[i]);
+          if (cut.size() == 0) {
+            for (int k = 0; k < nd; ++k) host[(size_t)k * npos + ip] += w * (float)blank[k];
+            ++ip;
+            continue;
+          }
+          const int S = set_specimen(plan.p, cut, EM_BORDER_MIN);
+          V.emplace_back(new Dev((size_t)S * N * 4));
+          ok(em_potential_slices(plan.p, 0, S, V.back()->as<float>()), plan.p, "em_potential_slices");
+          const double xb = 0, yb = 0;
+          ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, psi.as<float>() + 2 * N * pos.size()),
+             plan.p, "em_build_probes");
+          Vp.push_back(V.back()->as<float>());
+          Ss.push_back(S);
+          pos.push_back(ip);
+          used += (size_t)S * N * 4;
+          ++ip;
         }
-        const int S = set_specimen(plan.p, cut, EM_BORDER_MIN);
-        Dev V((size_t)S * N * 4);
-        ok(em_potential_slices(plan.p, 0, S, V.as<float>()), plan.p, "em_potential_slices");
-        const double xb = 0, yb = 0;
-        ok(em_build_probes(plan.p, &cond, zmid0(plan.p), &xb, &yb, 1, psi.p), plan.p, "em_build_probes");
-        ok(em_multislice_forward(plan.p, V.as<float>(), nullptr, S, psi.p, 1, nullptr, 1), plan.p,
-           "em_multislice_forward");
-        ok(em_detector_sums(plan.p, psi.p, 1, nd, in.data(), ou.data(), w, 0, d.as<float>()), plan.p,
+        const int k = (int)pos.size();
+        if (k == 0) continue;
+        ok(em_multislice_forward_multi(plan.p, Vp.data(), Ss.data(), psi.p, k), plan.p,
+           "em_multislice_forward_multi");
+        ok(em_detector_sums(plan.p, psi.p, k, nd, in.data(), ou.data(), w, 0, d.as<float>()), plan.p,
            "em_detector_sums");
-        hip_ok(hipMemcpy(hd.data(), d.p, sizeof(float) * nd, hipMemcpyDeviceToHost), "copy");
-        for (int k = 0; k < nd; ++k) host[(size_t)k * npos + ip] += hd[k];
+        hip_ok(hipMemcpy(hd.data(), d.p, sizeof(float) * nd * k, hipMemcpyDeviceToHost), "copy");
+        for (int b = 0; b < k; ++b)
+          for (int kk = 0; kk < nd; ++kk) host[(size_t)kk * npos + pos[b]] += hd[(size_t)kk * k + b];
       }
     }
   }
