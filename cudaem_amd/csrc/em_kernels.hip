@@ -891,8 +891,18 @@ __global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
     const float4* __restrict__ fine, float* V) {
   __shared__ float tab[kMaxTypes * 5 * kMaxNR];
   __shared__ float srm2[kMaxTypes];
+  // the R^2 nodes are log-spaced (r_min e^{i dl}, the last one r_max): the
+  // interval of r^2 starts at the node index of log2(r^2), then a bounded
+  // walk makes it exactly the one a binary search over the nodes finds
+  __shared__ float slog0[kMaxTypes], sinv[kMaxTypes];
   for (int i = threadIdx.x; i < ntypes * 5 * nR; i += blockDim.x) tab[i] = tables[i];
-  if (threadIdx.x < ntypes) srm2[threadIdx.x] = rmax2[threadIdx.x];
+  if (threadIdx.x < ntypes) {
+    const float* R2t = tables + threadIdx.x * 5 * nR;
+    srm2[threadIdx.x] = rmax2[threadIdx.x];
+    const float l0 = log2f(R2t[0]), l1 = log2f(R2t[nR - 1]);
+    slog0[threadIdx.x] = l0;
+    sinv[threadIdx.x] = l1 > l0 ? (float)(nR - 1) / (l1 - l0) : 0.f;
+  }
   __syncthreads();
   const int s = blockIdx.z;
   const int ix = blockIdx.x * kPotTile + (threadIdx.x % kPotTile);
@@ -919,11 +929,13 @@ __global__ __launch_bounds__(kPotTile* kPotTile) void potential_kernel(
       const int t = __float_as_int(at.w);
       if (r2 < srm2[t]) {
         const float* R2 = tab + t * 5 * nR;
-        int i0 = 0, ie = nR - 1;
-        while (ie - i0 > 1) {
-          const int im = (i0 + ie) >> 1;
-          if (r2 < R2[im]) ie = im; else i0 = im;
-        }
+        // i0: the largest node in [0, nR - 2] with R2[i0] <= r2, else 0
+        // (the binary search of get_value_Cubic_Spline,
+        // basic_functions_dev.cuh:49-75)
+        const float gi = (__log2f(r2) - slog0[t]) * sinv[t];
+        int i0 = gi > 0.f ? min((int)gi, nR - 2) : 0;
+        while (i0 > 0 && r2 < R2[i0]) --i0;
+        while (i0 < nR - 2 && !(r2 < R2[i0 + 1])) ++i0;
         const float dx = r2 - R2[i0];
         float val;
         if constexpr (FINE) {
