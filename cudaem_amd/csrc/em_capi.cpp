@@ -20,7 +20,7 @@ using namespace em;
 namespace {
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
-constexpr double kZEps = 1e-4;  // epsilon_rel of the slicer (float eps of misc_dev.cuh:272-273)
+constexpr double kZEps = 1e-4;  // epsilon_rel of the slicer (float eps of misc_dev.cuh:17-18)
 
 struct DevBuf {
   void* p = nullptr;
@@ -31,6 +31,9 @@ struct DevBuf {
   hipError_t ensure(size_t b) {
     if (b <= bytes && p) return hipSuccess;
     if (p) {
+      // work of an earlier call may still read the old buffer (em_stem
+      // returns with its graphs in flight)
+      (void)hipDeviceSynchronize();
       (void)hipFree(p);
       p = nullptr;
       bytes = 0;
@@ -51,6 +54,37 @@ struct Graph {
   ~Graph() {
     if (x) (void)hipGraphExecDestroy(x);
     if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+// Everything a captured STEM batch bakes into its launches: the batch shape,
+// detector bands, launch shapes, band skip and the device buffers it reads
+// and writes.  Zero-filled before use, so it compares bytewise.
+struct StemGraphKey {
+  int nb, S, ndet, skip_lo, skip_hi;
+  em_tuning tune;
+  double dg2min[kMaxDet], dg2max[kMaxDet];
+  const void* buf[10];
+  bool operator==(const StemGraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+// Pinned host memory (hipHostMalloc), for asynchronous uploads.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) {
+      (void)hipHostFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    return e;
   }
 };
 
@@ -124,6 +158,14 @@ struct em_plan_s {
   DevBuf mchunks;  // chunk tables of em_multislice_forward_multi
   int vstore_slices = -1;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
+  // em_stem: instantiated batch graphs kept across calls (a phonon step
+  // re-launches them), the probe positions of a call staged in pinned memory
+  // and uploaded once, and the events that order the graph stream after the
+  // caller's stream and back
+  std::vector<std::pair<StemGraphKey, std::unique_ptr<Graph>>> stem_graphs;
+  PinnedBuf pos_host;
+  DevBuf pos_dev;
+  hipEvent_t pos_copied = nullptr, stem_in = nullptr, stem_out = nullptr;
   // per-launch timing of the STEM slice loop's row / column passes: event
   // pairs (start, end) of each timed launch, kind 0 = row, 1 = column
   bool pass_timing = false;
@@ -135,7 +177,11 @@ struct em_plan_s {
     tkind.clear();
   }
   ~em_plan_s() {
+    if (gstream) (void)hipStreamSynchronize(gstream);
     clear_pass_events();
+    stem_graphs.clear();
+    for (hipEvent_t e : {pos_copied, stem_in, stem_out})
+      if (e) (void)hipEventDestroy(e);
     if (gstream) (void)hipStreamDestroy(gstream);
   }
 
@@ -428,6 +474,8 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
   p->gdev = make_grid_dev(*grid);
   EM_HIP(p.get(), prepare_col_kernels(grid->ny));
   EM_HIP(p.get(), hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&p->pos_copied, &p->stem_in, &p->stem_out})
+    EM_HIP(p.get(), hipEventCreateWithFlags(e, hipEventDisableTiming));
   EM_TRY(upload_twiddles(p.get(), p->tw_x, grid->nx));
   EM_TRY(upload_twiddles(p.get(), p->tw_y, grid->ny));
   // Propagator, built once per plan (wave_function.cuh:42-44)
@@ -458,6 +506,7 @@ em_status em_plan_set_band_skip(em_plan plan, double threshold, int* skip_lo, in
 em_status em_plan_destroy(em_plan plan) {
   if (!plan) return EM_ERR_INVALID;
   if (plan->stream) (void)hipStreamSynchronize(plan->stream);
+  if (plan->gstream) (void)hipStreamSynchronize(plan->gstream);
   delete plan;
   return EM_OK;
 }
@@ -1204,13 +1253,12 @@ em_status ensure_specimen_potential(em_plan p) {
 }
 
 // The body of one STEM probe batch (scan_simulator.cuh:88-109): probes of the
-// batch (already staged in xe / ye) are processed in groups; each group builds
-// its probes, runs every slice and leaves detector partials in dpart.
+// batch (their position ramps already in ex / ey) are processed in groups;
+// each group builds its probes, runs every slice and leaves detector partials
+// in dpart.
 em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const double* dg2max, int ndet) {
   const long long N = p->N();
   const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
-  EM_HIP(p, launch_probe_ramps(p->gdev, nb, p->xe.as<double>(), p->ye.as<double>(), p->ex.as<float2>(),
-                               p->ey.as<float2>(), p->stream));
   const int G = group_size(p, nb);
   float2* wave = p->wave.as<float2>();
   for (int g0 = 0; g0 < nb; g0 += G) {
@@ -1221,7 +1269,10 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     c.ey = p->ey.as<float2>() + (long long)g0 * p->grid.ny;
     EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
     for (int s = 0; s < S; ++s) {
-      RowArgs r = row_args_skip(p, wave, wave, ng, true, s < S - 1);
+      // slice 0 reads every column of the freshly built probe (the condenser
+      // aperture may reach past the skipped band); later slices read the
+      // output of a propagator pass, zero there
+      RowArgs r = row_args_skip(p, wave, wave, ng, s > 0, s < S - 1);
       unsigned rm;
       if (p->tune.row_variant == 2) {
         // the potential slice (4 B per pixel) with sincos once per workgroup
@@ -1274,69 +1325,94 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
     dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
     dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
   }
-  // every buffer the graph touches is sized before capture
+  // every buffer the graphs touch is sized before capture
   EM_HIP(p, p->wave.ensure(sizeof(float2) * N * B));
   EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)B * ndet * ntiles));
   EM_HIP(p, p->A.ensure(sizeof(float2) * N));
-  EM_HIP(p, p->xe.ensure(sizeof(double) * B));
-  EM_HIP(p, p->ye.ensure(sizeof(double) * B));
   EM_HIP(p, p->ex.ensure(sizeof(float2) * B * p->grid.nx));
   EM_HIP(p, p->ey.ensure(sizeof(float2) * B * p->grid.ny));
-  // order the private graph stream after the caller's stream
-  EventSet evs;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  EM_HIP(p, evs.make(&ev0, hipEventDisableTiming));
-  EM_HIP(p, evs.make(&ev1, hipEventDisableTiming));
-  EM_HIP(p, hipEventRecord(ev0, p->stream));
-  EM_HIP(p, hipStreamWaitEvent(p->gstream, ev0, 0));
+  EM_HIP(p, p->pos_dev.ensure(sizeof(double) * 2 * (size_t)npos));
+  // Probe phase ramps of every position (xe = -2 pi (x_b - x_l/2),
+  // grid_dev.cuh:104-111) staged in pinned memory and uploaded once per
+  // call; batches read them at their offset.  The previous call's upload must
+  // have left the staging buffer first.
+  EM_HIP(p, hipEventSynchronize(p->pos_copied));
+  EM_HIP(p, p->pos_host.ensure(sizeof(double) * 2 * (size_t)npos));
+  double* ph = reinterpret_cast<double*>(p->pos_host.p);
+  for (int b = 0; b < npos; ++b) {
+    ph[b] = -2.0 * kPi * (x[b] - 0.5 * p->grid.lx);
+    ph[npos + b] = -2.0 * kPi * (y[b] - 0.5 * p->grid.ly);
+  }
+  const double* xe = p->pos_dev.as<double>();
+  const double* ye = xe + npos;
+  // the private graph stream runs after the caller's stream, and the
+  // caller's stream after it: no host synchronisation in the call
   const hipStream_t user = p->stream;
+  EM_HIP(p, hipEventRecord(p->stem_in, user));
+  EM_HIP(p, hipStreamWaitEvent(p->gstream, p->stem_in, 0));
   em_status st = EM_OK;
   {
     StreamSwap sw(p, p->gstream);
+    st = p->hip(hipMemcpyAsync(p->pos_dev.p, p->pos_host.p, sizeof(double) * 2 * (size_t)npos,
+                               hipMemcpyHostToDevice, p->stream), "positions");
+    if (st == EM_OK) st = p->hip(hipEventRecord(p->pos_copied, p->stream), "record");
     // condenser-lens table with the defocus moved to the middle of slice 0
     // (incident_wave.cuh:33-35)
     const double zmid0 = 0.5 * (p->slices[0].z_start + p->slices[0].z_end);
-    st = p->hip(launch_lens_table(p->gdev, derive_lens(*cond, p->k.lambda, zmid0), p->A.as<float2>(), p->stream),
-                "lens table");
-    std::map<int, std::unique_ptr<Graph>> graphs;  // per batch size, this call only
-    std::vector<double> xe(B), ye(B);
+    if (st == EM_OK)
+      st = p->hip(launch_lens_table(p->gdev, derive_lens(*cond, p->k.lambda, zmid0), p->A.as<float2>(), p->stream),
+                  "lens table");
     for (int start = 0; start < npos && st == EM_OK; start += B) {
       const int nb = std::min(B, npos - start);
-      for (int b = 0; b < nb; ++b) {
-        xe[b] = -2.0 * kPi * (x[start + b] - 0.5 * p->grid.lx);
-        ye[b] = -2.0 * kPi * (y[start + b] - 0.5 * p->grid.ly);
-      }
-      // the previous batch has finished reading xe / ye once the stream drains
-      if ((st = p->hip(hipStreamSynchronize(p->stream), "sync")) != EM_OK) break;
-      if ((st = p->hip(hipMemcpy(p->xe.p, xe.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "xe")) != EM_OK) break;
-      if ((st = p->hip(hipMemcpy(p->ye.p, ye.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "ye")) != EM_OK) break;
+      if ((st = p->hip(launch_probe_ramps(p->gdev, nb, xe + start, ye + start, p->ex.as<float2>(),
+                                          p->ey.as<float2>(), p->stream), "probe ramps")) != EM_OK)
+        break;
       if (p->pass_timing) {
         // timed launches run eagerly: events between the passes need a plain stream
         if ((st = stem_batch_body(p, nb, S, dg2min, dg2max, ndet)) != EM_OK) break;
-        st = p->hip(launch_det_reduce(nb, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out + start, npos,
-                                      p->stream),
-                    "detector reduce");
-        continue;
+      } else {
+        StemGraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.nb = nb;
+        key.S = S;
+        key.ndet = ndet;
+        key.skip_lo = p->skip_lo;
+        key.skip_hi = p->skip_hi;
+        key.tune = p->tune;
+        for (int d = 0; d < ndet; ++d) {
+          key.dg2min[d] = dg2min[d];
+          key.dg2max[d] = dg2max[d];
+        }
+        const void* bufs[10] = {p->wave.p, p->dpart.p, p->A.p, p->ex.p, p->ey.p, p->Tstore.p,
+                                p->Vstore.p, p->P.p, p->tw_x.p, p->tw_y.p};
+        std::memcpy(key.buf, bufs, sizeof(bufs));
+        Graph* graph = nullptr;
+        for (auto& e : p->stem_graphs)
+          if (e.first == key) graph = e.second.get();
+        if (!graph) {
+          if (p->stem_graphs.size() >= 8) p->stem_graphs.erase(p->stem_graphs.begin());
+          auto g = std::make_unique<Graph>();
+          if ((st = p->hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture")) != EM_OK)
+            break;
+          const em_status body = stem_batch_body(p, nb, S, dg2min, dg2max, ndet);
+          const hipError_t ec = hipStreamEndCapture(p->stream, &g->g);
+          if (body != EM_OK) {
+            st = body;
+            break;
+          }
+          if ((st = p->hip(ec, "end capture")) != EM_OK) break;
+          if ((st = p->hip(hipGraphInstantiate(&g->x, g->g, nullptr, nullptr, 0), "instantiate")) != EM_OK) break;
+          graph = g.get();
+          p->stem_graphs.emplace_back(key, std::move(g));
+        }
+        if ((st = p->hip(hipGraphLaunch(graph->x, p->stream), "graph launch")) != EM_OK) break;
       }
-      auto it = graphs.find(nb);
-      if (it == graphs.end()) {
-        auto g = std::make_unique<Graph>();
-        if ((st = p->hip(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "begin capture")) != EM_OK) break;
-        const em_status body = stem_batch_body(p, nb, S, dg2min, dg2max, ndet);
-        const hipError_t ec = hipStreamEndCapture(p->stream, &g->g);
-        if (body != EM_OK) { st = body; break; }
-        if ((st = p->hip(ec, "end capture")) != EM_OK) break;
-        if ((st = p->hip(hipGraphInstantiate(&g->x, g->g, nullptr, nullptr, 0), "instantiate")) != EM_OK) break;
-        it = graphs.emplace(nb, std::move(g)).first;
-      }
-      if ((st = p->hip(hipGraphLaunch(it->second->x, p->stream), "graph launch")) != EM_OK) break;
       st = p->hip(launch_det_reduce(nb, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out + start, npos, p->stream),
                   "detector reduce");
     }
-    if (st == EM_OK) st = p->hip(hipStreamSynchronize(p->stream), "sync");
   }
-  (void)hipEventRecord(ev1, p->gstream);
-  (void)hipStreamWaitEvent(user, ev1, 0);
+  (void)hipEventRecord(p->stem_out, p->gstream);
+  (void)hipStreamWaitEvent(user, p->stem_out, 0);
   return st;
 }
 

@@ -13,7 +13,7 @@
 //
 // Directions: DIR = -1 forward exp(-2 pi i nk/N), DIR = +1 inverse, both
 // unnormalised, matching the C2C convention the reference relies on
-// (fft_dev.cuh:233-264).
+// (fft_dev.cuh:159-190).
 #pragma once
 #include <hip/hip_runtime.h>
 

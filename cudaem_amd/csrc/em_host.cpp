@@ -120,28 +120,52 @@ Spline build_spline(const ElementCoefs& k, int nR, double r_min, double r_max, d
 }
 
 // Slice_Generator::gene_bounds / get_slicing / fd_by_z
-// (slice_generator.hpp:81-168), direct mode: slice k owns z_start <= z < z_end.
-std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int border, double eps) {
+// (slice_generator.hpp:81-168) in the real type T of the generator: the
+// reference instantiates only Slice_Generator<float> (specimen.cuh:57 through
+// the float simulators of simulator.cu:7-49), so the bounds, eps and every
+// membership comparison are float there, and an atom within a float rounding
+// of a bound lands in the slice the float arithmetic puts it in.
+// Direct mode: slice k takes the atoms with z_start <= z < z_end.  Fine mode
+// (:88-93, 108-128): bounds over [z_min - r_int, z_max + r_int] (the
+// integration range Atom_Coord_Data reports, from a missing header: r_int =
+// the largest spline radius), and slice k takes the atoms with z in
+// [min(z_mid - r_int, z_start) - eps, max(z_mid + r_int, z_end) + eps]
+// (Inc_Borders).
+namespace {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+template <typename T>
+std::vector<SliceRange> slice_atoms(const std::vector<T>& z, T dz, int border, T eps, bool fine, T r_int) {
   std::vector<SliceRange> out;
   const long long n = (long long)z.size();
   if (n == 0 || !(dz > 0)) return out;
-  const double vmin = z.front(), vmax = z.back();
-  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
-  std::vector<double> b(nv);
+  const T vmin = fine ? T(z.front() - r_int) : z.front();
+  const T vmax = fine ? T(z.back() + r_int) : z.back();
+  const T span = vmax - vmin;
+  const int nv = static_cast<int>(std::floor(T(span + eps) / dz)) + 2;
+  std::vector<T> b(nv);
   if (border == EM_BORDER_MIN) {
-    for (int i = 0; i < nv; ++i) b[i] = (vmin - 2 * eps) + i * dz;
+    const T zb = vmin - T(2) * eps;
+    for (int i = 0; i < nv; ++i) b[i] = zb + T(i) * dz;
   } else if (border == EM_BORDER_MAX) {
-    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = (vmax + 2 * eps) - i * dz;
+    const T zt = vmax + T(2) * eps;
+    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = zt - T(i) * dz;
   } else {
-    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
-    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
+    const T zb = T(0.5) * (T(vmin + vmax) - T(T(nv - 1) * dz));
+    for (int i = 0; i < nv; ++i) b[i] = zb + T(i) * dz;
   }
   out.resize(nv - 1);
   for (int s = 0; s < nv - 1; ++s) {
     SliceRange& r = out[s];
     r.z_start = b[s];
     r.z_end = b[s + 1];
-    const double zs = b[s], ze = b[s + 1];
+    T zs = b[s], ze = b[s + 1];
+    if (fine) {
+      const T zm = T(0.5) * (b[s] + b[s + 1]);
+      zs = T(std::fmin(T(zm - r_int), b[s]) - eps);
+      ze = T(std::fmax(T(zm + r_int), b[s + 1]) + eps);
+    }
     if (zs > ze || z[n - 1] < zs || ze < z[0]) {
       r.i0 = 1;
       r.i1 = 0;
@@ -157,47 +181,20 @@ std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int
   return out;
 }
 
-// Fine-sliced Slice_Generator (slice_generator.hpp:88-93, 108-128): bounds
-// over [z_min - r_int, z_max + r_int] (the integration range Atom_Coord_Data
-// reports, from a missing header: r_int = the largest spline radius), and
-// slice k takes the atoms with z in [min(z_mid - r_int, z_start) - eps,
-// max(z_mid + r_int, z_end) + eps] (Inc_Borders).
+std::vector<float> to_float(const std::vector<double>& z) {
+  std::vector<float> f(z.size());
+  for (size_t i = 0; i < z.size(); ++i) f[i] = (float)z[i];
+  return f;
+}
+}  // namespace
+
+std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int border, double eps) {
+  return slice_atoms<float>(to_float(z), (float)dz, border, (float)eps, false, 0.f);
+}
+
 std::vector<SliceRange> make_slices_fine(const std::vector<double>& z, double dz, int border, double eps,
                                          double r_int) {
-  std::vector<SliceRange> out;
-  const long long n = (long long)z.size();
-  if (n == 0 || !(dz > 0)) return out;
-  const double vmin = z.front() - r_int, vmax = z.back() + r_int;
-  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
-  std::vector<double> b(nv);
-  if (border == EM_BORDER_MIN) {
-    for (int i = 0; i < nv; ++i) b[i] = (vmin - 2 * eps) + i * dz;
-  } else if (border == EM_BORDER_MAX) {
-    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = (vmax + 2 * eps) - i * dz;
-  } else {
-    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
-    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
-  }
-  out.resize(nv - 1);
-  for (int s = 0; s < nv - 1; ++s) {
-    SliceRange& r = out[s];
-    r.z_start = b[s];
-    r.z_end = b[s + 1];
-    const double zm = 0.5 * (b[s] + b[s + 1]);
-    const double zs = std::fmin(zm - r_int, b[s]) - eps, ze = std::fmax(zm + r_int, b[s + 1]) + eps;
-    if (zs > ze || z[n - 1] < zs || ze < z[0]) {
-      r.i0 = 1;
-      r.i1 = 0;
-      continue;
-    }
-    r.i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z.begin(), z.end(), zs) - z.begin());
-    r.i1 = (z[n - 1] <= ze) ? n - 1 : (long long)(std::lower_bound(z.begin(), z.end(), ze) - z.begin()) - 1;
-    if (r.i0 > r.i1 || z[r.i1] < zs || ze < z[r.i0]) {
-      r.i0 = 1;
-      r.i1 = 0;
-    }
-  }
-  return out;
+  return slice_atoms<float>(to_float(z), (float)dz, border, (float)eps, true, (float)r_int);
 }
 
 // Newton iteration on the Legendre polynomial P_n from the usual cosine

@@ -32,7 +32,9 @@ struct SliceRange {
   double z_start, z_end;
   long long i0, i1;  // inclusive atom range; i0 > i1 when empty
 };
-// z sorted ascending.
+// z sorted ascending.  Slicing runs in float, as Slice_Generator<float> does:
+// z, dz, eps and r_int are rounded to float and every bound and comparison is
+// float; the bounds are returned as the float values, widened to double.
 std::vector<SliceRange> make_slices(const std::vector<double>& z, double dz, int border, double eps);
 // Fine-sliced mode: bounds over [z_min - r_int, z_max + r_int]; slice k holds
 // the atoms within r_int of its centre (and those inside it), widened by eps.

@@ -16,7 +16,7 @@
  *    asynchronous unless the function returns host data.
  *  - One plan per host thread or stream; a plan is not thread-safe.
  *  - FFT convention: forward exp(-2 pi i nk/N), inverse exp(+2 pi i nk/N),
- *    both unnormalised (fft_dev.cuh:233-264).  Exit waves are left in Fourier
+ *    both unnormalised (fft_dev.cuh:159-190).  Exit waves are left in Fourier
  *    space, unnormalised (wave_function.cuh:60-73).
  */
 #ifndef CUDAEM_AMD_H

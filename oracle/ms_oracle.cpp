@@ -8,7 +8,8 @@
 // orc_stem_f32 runs the STEM loop in FP32 as a second CPU timing; it is
 // checked against the FP64 loop and is not an oracle.
 //
-// Every function restates the reference's algorithm in double precision and
+// Every function restates the reference's algorithm in double precision
+// (slicing, which is index work, in float as the reference does it) and
 // cites the reference file:line it follows (the reference is a flat directory
 // of CUDA headers).  Pieces the reference does not ship (element tables, the
 // spline builder, sigma, the aperture conversion, the phonon RNG) are fixed
@@ -19,7 +20,7 @@
 //   * complex arrays are interleaved (re, im) doubles, row-major [row][col],
 //     col (x) fastest;
 //   * forward FFT = exp(-2 pi i nk/N), inverse = exp(+2 pi i nk/N), both
-//     unnormalised (fft_dev.cuh:233-264);
+//     unnormalised (fft_dev.cuh:159-190);
 //   * pixel (ix, iy) sits at (ix*dx, iy*dy); frequencies use the unshifted
 //     index i < n/2 ? i : i - n (grid_dev.cuh:160-175).
 
@@ -397,6 +398,22 @@ struct FftT {
         const C tw = sign < 0 ? w[e] : std::conj(w[e]);
         tmp[r] = out[r * q + k] * tw;
       }
+      if (p == 2) {
+        out[k] = tmp[0] + tmp[1];
+        out[k + q] = tmp[0] - tmp[1];
+        continue;
+      }
+      if (p == 4) {
+        // W_4 = exp(sign i pi / 2) = sign * i
+        const C t0 = tmp[0] + tmp[2], t1 = tmp[0] - tmp[2], t2 = tmp[1] + tmp[3];
+        const C d = tmp[1] - tmp[3];
+        const C t3 = sign < 0 ? C(d.imag(), -d.real()) : C(-d.imag(), d.real());
+        out[k] = t0 + t2;
+        out[k + q] = t1 + t3;
+        out[k + 2 * q] = t0 - t2;
+        out[k + 3 * q] = t1 - t3;
+        continue;
+      }
       for (int s = 0; s < p; ++s) {
         C acc(0, 0);
         for (int r = 0; r < p; ++r) {
@@ -471,7 +488,7 @@ struct Lens {
   double m, cc10, cc12, cc21, cc23, cc30, cc32, cc34, cc41, cc43, cc45, cc50, cc52, cc54, cc56;
   double phi12, phi21, phi23, phi32, phi34, phi41, phi43, phi45, phi52, phi54, phi56;
   double g2_min, g2_max;
-  // isZero_dev<float> gate (misc_dev.cuh:268-278, used via nonZero_dev in lens_dev.cuh:165-252)
+  // isZero_dev<float> gate (isEqual_dev<float>, misc_dev.cuh:14-24, 36-53; used via nonZero_dev in lens_dev.cuh:165-252)
   static bool nz(double v) { return std::fabs(v) > 1e-5; }
   Lens(const double* p, double lambda, double defocus_shift) {
     const double l = lambda, l2 = l * l, l3 = l2 * l, l4 = l3 * l, l5 = l4 * l;
@@ -573,38 +590,59 @@ void probe(const Grid& g, const Lens& lens, double xb, double yb, cd* psi) {
 
 // ---------------------------------------------------------------------------
 // Slicing, Slice_Generator::gene_bounds / get_slicing / fd_by_z
-// (slice_generator.hpp:81-168), direct (non fine-sliced) mode: slice k owns the
-// atoms with z_start <= z < z_end.  border: 0 = eMB_Min (STEM/ED/CBED),
-// 1 = eMB_Max (HRTEM), 2 = eMB_Middle (specimen.cuh:24-31).  eps = epsilon_rel,
-// fixed by the build at 1e-4 (the float tolerance of misc_dev.cuh:272-273).
-// z must be sorted ascending.  Returns the number of slices.
+// (slice_generator.hpp:81-168).  The reference instantiates the generator
+// only for float (Specimen<float>, specimen.cuh:57, through simulator.cu:7-49),
+// so slicing is index work done in float arithmetic: z, dz, eps and R_int are
+// rounded to float, the bounds are formed with float operations in the order
+// gene_bounds writes them, and every membership test compares floats.
+// Direct mode: slice k owns the atoms with z_start <= z < z_end.  Fine mode
+// (:88-93, 108-128): bounds cover [z_min - R_int, z_max + R_int] (the
+// integration range of Atom_Coord_Data's statistics, in a missing header: the
+// build takes R_int = the largest spline radius), and slice k takes the atoms
+// with z in [min(z_mid - R_int, z_start), max(z_mid + R_int, z_end)] widened
+// by eps on both sides (Inc_Borders).  border: 0 = eMB_Min (STEM/ED/CBED),
+// 1 = eMB_Max (HRTEM), 2 = eMB_Middle (specimen.cuh:24-31).  eps =
+// epsilon_rel, fixed by the build at 1e-4 (the float tolerance of
+// misc_dev.cuh:17-18).  z must be sorted ascending.  Returns the number of
+// slices.
 // ---------------------------------------------------------------------------
-int slicing(int n, const double* z, double dz, int border, double eps, int max_slices,
-            double* bounds, long long* istart, long long* iend) {
+template <typename T>
+int slicing_t(int n, const double* zd, double dz_d, int border, double eps_d, bool fine, double r_int_d,
+              int max_slices, double* bounds, long long* istart, long long* iend) {
   if (n <= 0) return 0;
-  const double vmin = z[0], vmax = z[n - 1];
-  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
+  std::vector<T> z(n);
+  for (int i = 0; i < n; ++i) z[i] = (T)zd[i];
+  const T dz = (T)dz_d, eps = (T)eps_d, r_int = (T)r_int_d;
+  const T vmin = fine ? T(z[0] - r_int) : z[0];
+  const T vmax = fine ? T(z[n - 1] + r_int) : z[n - 1];
+  const T s_v = vmax - vmin;
+  const int nv = (int)std::floor(T(s_v + eps) / dz) + 2;
   if (nv - 1 > max_slices) return -(nv - 1);
-  std::vector<double> b(nv);
+  std::vector<T> b(nv);
   if (border == 0) {
-    const double zb = vmin - 2 * eps;
-    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
+    const T zb = vmin - T(2) * eps;
+    for (int i = 0; i < nv; ++i) b[i] = zb + T(i) * dz;
   } else if (border == 1) {
-    const double zt = vmax + 2 * eps;
-    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = zt - i * dz;
+    const T zt = vmax + T(2) * eps;
+    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = zt - T(i) * dz;
   } else {
-    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
-    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
+    const T zb = T(0.5) * (T(vmin + vmax) - T(T(nv - 1) * dz));
+    for (int i = 0; i < nv; ++i) b[i] = zb + T(i) * dz;
   }
   for (int i = 0; i < nv; ++i) bounds[i] = b[i];
   for (int s = 0; s < nv - 1; ++s) {
-    const double zs = b[s], ze = b[s + 1];
+    T zs = b[s], ze = b[s + 1];
+    if (fine) {
+      const T zm = T(0.5) * (b[s] + b[s + 1]);
+      zs = T(std::fmin(T(zm - r_int), b[s]) - eps);
+      ze = T(std::fmax(T(zm + r_int), b[s + 1]) + eps);
+    }
     long long i0, i1;
     if (zs > ze || z[n - 1] < zs || ze < z[0]) {
       i0 = 1; i1 = 0;
     } else {
-      i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z, z + n, zs) - z);
-      i1 = (z[n - 1] <= ze) ? (n - 1) : (long long)(std::lower_bound(z, z + n, ze) - z) - 1;
+      i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z.begin(), z.end(), zs) - z.begin());
+      i1 = (z[n - 1] <= ze) ? (n - 1) : (long long)(std::lower_bound(z.begin(), z.end(), ze) - z.begin()) - 1;
       if (i0 > i1 || z[i1] < zs || ze < z[i0]) { i0 = 1; i1 = 0; }
     }
     istart[s] = i0;
@@ -613,43 +651,14 @@ int slicing(int n, const double* z, double dz, int border, double eps, int max_s
   return nv - 1;
 }
 
-// Fine-sliced mode of the same generator (slice_generator.hpp:88-93,
-// 108-128): bounds cover [z_min - R_int, z_max + R_int] (the integration range
-// of Atom_Coord_Data's statistics, in a missing header: the build takes
-// R_int = the largest spline radius), and slice k takes the atoms with z in
-// [min(z_mid - R_int, z_start), max(z_mid + R_int, z_end)] widened by eps on
-// both sides (Inc_Borders).
+int slicing(int n, const double* z, double dz, int border, double eps, int max_slices,
+            double* bounds, long long* istart, long long* iend) {
+  return slicing_t<float>(n, z, dz, border, eps, false, 0.0, max_slices, bounds, istart, iend);
+}
+
 int slicing_fine(int n, const double* z, double dz, int border, double eps, double r_int,
                  int max_slices, double* bounds, long long* istart, long long* iend) {
-  if (n <= 0) return 0;
-  const double vmin = z[0] - r_int, vmax = z[n - 1] + r_int;
-  const int nv = (int)std::floor((vmax - vmin + eps) / dz) + 2;
-  if (nv - 1 > max_slices) return -(nv - 1);
-  std::vector<double> b(nv);
-  if (border == 0) {
-    for (int i = 0; i < nv; ++i) b[i] = (vmin - 2 * eps) + i * dz;
-  } else if (border == 1) {
-    for (int i = 0; i < nv; ++i) b[nv - 1 - i] = (vmax + 2 * eps) - i * dz;
-  } else {
-    const double zb = 0.5 * (vmin + vmax - (nv - 1) * dz);
-    for (int i = 0; i < nv; ++i) b[i] = zb + i * dz;
-  }
-  for (int i = 0; i < nv; ++i) bounds[i] = b[i];
-  for (int s = 0; s < nv - 1; ++s) {
-    const double zm = 0.5 * (b[s] + b[s + 1]);
-    const double zs = std::fmin(zm - r_int, b[s]) - eps, ze = std::fmax(zm + r_int, b[s + 1]) + eps;
-    long long i0, i1;
-    if (zs > ze || z[n - 1] < zs || ze < z[0]) {
-      i0 = 1; i1 = 0;
-    } else {
-      i0 = (zs <= z[0]) ? 0 : (long long)(std::lower_bound(z, z + n, zs) - z);
-      i1 = (z[n - 1] <= ze) ? (n - 1) : (long long)(std::lower_bound(z, z + n, ze) - z) - 1;
-      if (i0 > i1 || z[i1] < zs || ze < z[i0]) { i0 = 1; i1 = 0; }
-    }
-    istart[s] = i0;
-    iend[s] = i1;
-  }
-  return nv - 1;
+  return slicing_t<float>(n, z, dz, border, eps, true, r_int, max_slices, bounds, istart, iend);
 }
 
 // ---------------------------------------------------------------------------

@@ -54,6 +54,35 @@ int main(int argc, char** argv) {
   }
   const std::string mode = argv[1], out = argv[2];
   try {
+    if (mode == "config2") {
+      // Plain STEM ADF as the reference's general scanner runs it: 512^2,
+      // SrTiO3 10 x 10 x 40 cells, 80 slices, 300 kV, 25 mrad, Cs = 0,
+      // defocus 0, detector 60-200 mrad; positions (5, 3), (16, 3), (5, 16),
+      // (16, 16) of the 32 x 32 raster over the field.  mulSimulator always
+      // scans the whole field (simulator.cu:40-46), so a subset of the raster
+      // goes through scan_simulator with its own scan range, as in
+      // executionSTEM otherwise.
+      multislice_params<float> p = base(512, 10, 40);
+      p.simulation_type = eTEMST_STEM;
+      p.cond_lens.outer_aper_ang = 25;
+      p.detector.ang_inner = {60};
+      p.detector.ang_outer = {200};
+      p.STEM_col = 2;
+      p.STEM_row = 2;
+      p.stem_scanning = eGeneral_scanning;
+      p.is_crystal = false;
+      p.validate_parameters();
+      p.initial_CL();
+      p.set_Scanning({5.f / 32, 3.f / 32, 27.f / 32, 29.f / 32});
+      output_Multi_Data<float> o;
+      scan_simulator<float> sim;
+      sim.set_input_data(&p, &o);
+      sim.STEM_multiGPU();
+      write(out + "/config2_stem.f32", o.STEM_intensity);
+      write(out + "/config2_total.f32", {o.total_intensity});
+      std::printf("config2 %zu total %g\n", o.STEM_intensity.size(), o.total_intensity);
+      return 0;
+    }
     if (mode == "io") {  // text atoms in, raw float32 out and back (no GPU)
       const Atom_Coord_Data<float> at = read_atoms_text<float>(out + "/atoms.txt");
       std::vector<float> r;

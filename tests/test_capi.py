@@ -121,6 +121,38 @@ def test_host_slicing_matches_oracle(C, border):
             assert np.array_equal(b, bo) and np.array_equal(i0, i0o) and np.array_equal(i1, i1o)
 
 
+def _workload_specimens():
+    """Seeded phonon-displaced specimens of the four forward workloads, with
+    their slice thickness and border: Cu 8x8x10 (HRTEM, 1.8075 A, top
+    aligned), SrTiO3 10x10x40 (STEM, 1.9525 A, u_rms 0.08 A, the 2-probe
+    general scan and the batched phonon steps), Si diamond 50-400 A
+    (dz = a/4)."""
+    import cudaem_amd.specimen as sp
+    yield "cu", sp.vibrate(sp.fcc_cu(8, 8, 10), 0.08, 21, 1), 1.8075, 1
+    yield "cu-static", sp.fcc_cu(8, 8, 10), 1.8075, 1
+    for conf in (1, 2, 3, 4):
+        yield f"srtio3-{conf}", sp.vibrate(sp.srtio3(10, 10, 40), 0.08, 1234, conf), 1.9525, 0
+    for nz in (9, 37, 74):
+        yield f"si-{nz}", sp.vibrate(sp.si_diamond(5, 5, nz), 0.08, 7, nz), 5.431 / 4, 0
+
+
+def test_host_slicing_is_float_on_workload_specimens(C):
+    """The product's slicer against a brute-force float32 restatement, bit
+    for bit in bounds and every slice's (i0, i1), on the workloads' specimens;
+    and on each, with an atom planted where double arithmetic would put it in
+    another slice."""
+    from slicing_ref import brute_force_slicing, planted_atom
+    for name, at, dz, border in _workload_specimens():
+        z = np.sort(at["z"], kind="stable")
+        for zz in (z, np.sort(np.append(z, planted_atom(z, dz, border)))):
+            b, i0, i1 = _host_slicing(C, zz, dz, border)
+            bf, i0f, i1f = brute_force_slicing(zz, dz, border, O.Z_EPS)
+            assert np.array_equal(b, bf), name
+            assert np.array_equal(i0, i0f) and np.array_equal(i1, i1f), name
+        bd, i0d, i1d = brute_force_slicing(zz, dz, border, O.Z_EPS, real=np.float64)
+        assert not (np.array_equal(i0, i0d) and np.array_equal(i1, i1d)), name
+
+
 @pytest.mark.parametrize("border", [0, 1, 2])
 def test_host_slicing_fine_matches_oracle(C, border):
     """Fine-sliced bounds and atom windows equal the oracle's bit for bit."""

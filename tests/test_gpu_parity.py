@@ -835,3 +835,72 @@ def test_band_skip_is_below_fp32_resolution(em):
     assert rel_l2(res[1e-9][0], res[0.0][0]) < 1e-6
     assert rel_l2(res[1e-9][1], res[0.0][1]) < 1e-6
     assert rel_l2(res[1e-9][2], res[0.0][2]) < 1e-6
+
+
+def test_stem_graph_cache_follows_changes(em):
+    """em_stem keeps its captured batch graphs across calls: a repeated call
+    is bitwise identical; a new specimen is read by the kept graph; launch
+    shapes, band skip, detector bands and batch sizes that change what a
+    graph bakes in give the results a fresh plan gives."""
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 4)
+    n, dz, e0 = 128, 1.9525, 300.0
+    lens = probe_lens(25.0)
+    rng = np.random.default_rng(8)
+    xs, ys = rng.uniform(0, at["lx"], 7), rng.uniform(0, at["ly"], 7)
+    at2 = sp.vibrate(at, 0.1, 3, 1)
+
+    def fresh(atoms, inner, outer, **tun):
+        with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=4) as q:
+            if tun:
+                q.set_tuning(**tun)
+            q.set_specimen(atoms["x"], atoms["y"], atoms["z"], atoms["Z"], atoms["occ"], border=em.BORDER_MIN)
+            return _np(q.stem(lens, xs, ys, inner, outer))
+
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=4) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        r1 = _np(p.stem(lens, xs, ys, [40.0], [120.0]))  # batches of 4 and 3
+        assert np.array_equal(_np(p.stem(lens, xs, ys, [40.0], [120.0])), r1)
+        p.set_specimen(at2["x"], at2["y"], at2["z"], at2["Z"], at2["occ"], border=em.BORDER_MIN)
+        r2 = _np(p.stem(lens, xs, ys, [40.0], [120.0]))
+        assert rel_l2(r2, r1) > 1e-4
+        assert np.array_equal(r2, fresh(at2, [40.0], [120.0]))
+        r3 = _np(p.stem(lens, xs, ys, [0.0, 40.0], [40.0, 150.0]))
+        assert np.array_equal(r3, fresh(at2, [0.0, 40.0], [40.0, 150.0]))
+        p.set_tuning(probe_group=2, row_blocks=64)
+        assert np.array_equal(_np(p.stem(lens, xs, ys, [40.0], [120.0])),
+                              fresh(at2, [40.0], [120.0], probe_group=2, row_blocks=64))
+        p.set_band_skip(0.0)
+        assert rel_l2(_np(p.stem(lens, xs, ys, [40.0], [120.0])), r2) < 1e-6
+
+
+def test_stem_wide_aperture_across_skipped_band(em):
+    """A 120 mrad condenser aperture at 128^2 over 11.7 A reaches past the
+    band-limit skip (columns |kx| >= 55 of 64 here): the freshly built probe
+    has weight in those columns, and the first slice must transform all of
+    it, so the batched STEM sums still match the oracle (which skips nothing)
+    and the same plan with skipping off."""
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(3, 3, 4)
+    n, dz, e0 = 128, 1.9525, 300.0
+    lens = probe_lens(120.0)
+    xs = np.array([1.0, 4.2, 7.9, 10.3])
+    ys = np.array([0.5, 6.6, 3.3, 9.0])
+    inner, outer = np.array([0.0, 40.0]), np.array([40.0, 250.0])
+    res = {}
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=4) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        for thr in (0.0, 1e-9):
+            lo, hi = p.set_band_skip(thr)
+            if thr > 0:
+                assert 0 < lo <= 60 and hi == n - lo
+            res[thr] = _np(p.stem(lens, xs, ys, inner, outer))
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
+    # the aperture does cross the skipped columns
+    assert np.sin(120e-3) / o.lam > lo / at["lx"]
+    g2 = [(np.sin(a * 1e-3) / o.lam) ** 2 for a in (inner, outer)]
+    ref, _ = O.stem(n, n, at["lx"], at["ly"], o.lam, lens, o.zmid0(), o.propagator(), o.potentials(), xs, ys,
+                    g2[0], g2[1])
+    print("wide aperture STEM relL2", rel_l2(res[1e-9], ref), rel_l2(res[0.0], ref))
+    assert rel_l2(res[1e-9], res[0.0]) < 1e-6
+    assert rel_l2(res[1e-9], ref) < TOL_INT

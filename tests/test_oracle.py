@@ -139,23 +139,43 @@ def test_slice_counts_of_the_workloads():
 def test_slicing_partitions_atoms(border):
     rng = np.random.default_rng(3)
     z = np.sort(rng.uniform(-3, 17, 500))
+    z32 = z.astype(np.float32).astype(np.float64)  # the float atoms the slicer sees
     dz = 1.3
     b, i0, i1 = O.slicing(z, dz, border)
-    assert np.allclose(np.diff(b), dz)
+    assert np.allclose(np.diff(b), dz, rtol=1e-5)
+    assert np.array_equal(b.astype(np.float32).astype(np.float64), b)  # float bounds
     owner = np.full(len(z), -1)
     for s in range(len(i0)):
         if i0[s] <= i1[s]:
             idx = np.arange(i0[s], i1[s] + 1)
             assert np.all(owner[idx] == -1)
             owner[idx] = s
-            assert np.all((z[idx] >= b[s]) & (z[idx] < b[s + 1]))
+            assert np.all((z32[idx] >= b[s]) & (z32[idx] < b[s + 1]))
     # brute force: atom j belongs to the slice whose half-open range holds it
     for j in range(len(z)):
-        s = np.searchsorted(b, z[j], side="right") - 1
+        s = np.searchsorted(b, z32[j], side="right") - 1
         if 0 <= s < len(i0):
             assert owner[j] == s
     if border != O.BORDER_MIDDLE:
         assert np.all(owner >= 0)
+
+
+@pytest.mark.parametrize("border", [O.BORDER_MIN, O.BORDER_MAX, O.BORDER_MIDDLE])
+def test_slicing_is_float_arithmetic(border):
+    """Bounds and memberships equal a brute-force float32 restatement bit for
+    bit on phonon-displaced specimens, and an atom planted between a bound's
+    float and double values is placed where the float arithmetic puts it."""
+    import cudaem_amd.specimen as sp
+    from slicing_ref import brute_force_slicing, planted_atom
+    st = sp.vibrate(sp.srtio3(3, 3, 40), 0.08, 11, 2 + border)
+    z = np.sort(st["z"], kind="stable")
+    for zz in (z, np.sort(np.append(z, planted_atom(z, 1.9525, border)))):
+        b, i0, i1 = O.slicing(zz, 1.9525, border)
+        bf, i0f, i1f = brute_force_slicing(zz, 1.9525, border, O.Z_EPS)
+        assert np.array_equal(b, bf) and np.array_equal(i0, i0f) and np.array_equal(i1, i1f)
+        bd, i0d, i1d = brute_force_slicing(zz, 1.9525, border, O.Z_EPS, real=np.float64)
+    # with the planted atom, double arithmetic would slice differently
+    assert not (np.array_equal(i0, i0d) and np.array_equal(i1, i1d))
 
 
 # ----------------------------------------------------------------- operators
@@ -482,20 +502,26 @@ def test_fine_spline_taper_and_cubic():
 
 @pytest.mark.parametrize("border", [O.BORDER_MIN, O.BORDER_MAX, O.BORDER_MIDDLE])
 def test_slicing_fine_membership_brute_force(border):
+    from slicing_ref import brute_force_slicing
     rng = np.random.default_rng(7 + border)
     z = np.sort(rng.uniform(0, 20, 300))
     dz, r_int, eps = 1.35775, 3.0, O.Z_EPS
     b, i0, i1 = O.slicing_fine(z, dz, border, r_int)
     assert b[0] <= z[0] - r_int + 3 * eps and b[-1] >= z[-1] + r_int - 3 * eps
-    np.testing.assert_allclose(np.diff(b), dz, rtol=1e-12)
+    np.testing.assert_allclose(np.diff(b), dz, rtol=1e-5)
+    z32 = z.astype(np.float32)
+    f = np.float32
     for s in range(len(i0)):
-        zm = 0.5 * (b[s] + b[s + 1])
-        lo, hi = min(zm - r_int, b[s]) - eps, max(zm + r_int, b[s + 1]) + eps
-        idx = np.nonzero((z >= lo) & (z < hi))[0]
+        zm = f(0.5) * (f(b[s]) + f(b[s + 1]))
+        lo = f(min(f(zm - f(r_int)), f(b[s])) - f(eps))
+        hi = f(max(f(zm + f(r_int)), f(b[s + 1])) + f(eps))
+        idx = np.nonzero((z32 >= lo) & (z32 < hi))[0]
         if len(idx) == 0:
             assert i0[s] > i1[s]
         else:
             assert (i0[s], i1[s]) == (idx[0], idx[-1])
+    bf, i0f, i1f = brute_force_slicing(z, dz, border, eps, fine=True, r_int=r_int)
+    assert np.array_equal(b, bf) and np.array_equal(i0, i0f) and np.array_equal(i1, i1f)
 
 
 def test_potential_fine_brute_force_and_slice_sum():
