@@ -34,7 +34,7 @@ EXPORTS = [
     "em_plan_set_specimen", "em_slice_info",
     "em_potential_slices", "em_spline_tables", "em_propagator", "em_build_probes",
     "em_plane_wave", "em_fft2d", "em_multislice_forward", "em_multislice_forward_multi", "em_multislice_adjoint",
-    "em_detector_sums", "em_ctf_image", "em_pctf_image", "em_tcc_image", "em_diffraction", "em_stem", "em_thickness",
+    "em_detector_sums", "em_detector_sums_noalloc", "em_ctf_image", "em_pctf_image", "em_tcc_image", "em_diffraction", "em_stem", "em_thickness",
     "em_thickness_multi",
     "em_multislice_ragged", "em_time_passes", "em_host_slicing", "em_host_slicing_fine", "em_host_quadrature", "em_host_spline",
     "em_host_constants", "em_plan_get_tuning", "em_plan_set_tuning", "em_plan_autotune",
@@ -114,6 +114,7 @@ _SIGS = {
     "em_multislice_forward_multi": ([_P, _P, _P, _P, _I], _I),
     "em_multislice_adjoint": ([_P, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P], _I),
     "em_detector_sums": ([_P, _P, _I, _I, _PD, _PD, _F, _I, _P], _I),
+    "em_detector_sums_noalloc": ([_P, _P, _I, _I, _PD, _PD, _F, _I, _P], _I),
     "em_ctf_image": ([_P, ctypes.POINTER(em_lens), _P, _I, _P, _P, _P, _F, _I], _I),
     "em_pctf_image": ([_P, ctypes.POINTER(em_lens), ctypes.POINTER(em_coherence), _P, _I, _P, _P, _P, _F, _I], _I),
     "em_tcc_image": ([_P, ctypes.POINTER(em_lens), ctypes.POINTER(em_coherence), _P, _I, _P, _P, _P, _F, _I], _I),

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -108,12 +109,39 @@ struct EventSet {
   }
 };
 
+// Fixed device scratch of the allocation-free entry points: the detector
+// pass's per-tile partial sums, [nprobe][ndet][ntiles] floats.  Each device
+// holds its own copy until the process ends; host code uses the address
+// hipGetSymbolAddress reports for the current device, never the name.
+constexpr size_t kNoAllocScratchFloats = size_t(1) << 18;  // 1 MiB
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address, looked up once per device.
+hipError_t device_scratch(int dev, float** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 
 struct em_plan_s {
   em_grid grid{};
   em_beam beam{};
   int max_batch = 1;
+  int device = 0;
+  float* scratch = nullptr;  // this device's fixed scratch (allocation-free entry points)
   hipStream_t stream = nullptr;
   Constants k{};
   double vr_factor = 0, gx0 = 0, gy0 = 0;
@@ -472,6 +500,8 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
   p->gx0 = std::sin(beam->theta) * std::cos(beam->phi) / p->k.lambda;
   p->gy0 = std::sin(beam->theta) * std::sin(beam->phi) / p->k.lambda;
   p->gdev = make_grid_dev(*grid);
+  EM_HIP(p.get(), hipGetDevice(&p->device));
+  EM_HIP(p.get(), device_scratch(p->device, &p->scratch));
   EM_HIP(p.get(), prepare_col_kernels(grid->ny));
   EM_HIP(p.get(), hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
   for (hipEvent_t* e : {&p->pos_copied, &p->stem_in, &p->stem_out})
@@ -1116,6 +1146,33 @@ em_status em_detector_sums(em_plan p, const void* Psi, int nprobe, int ndet, con
   c.dpart = p->dpart.as<float>();
   EM_TRY(col(p, COL_DET, c));
   EM_HIP(p, launch_det_reduce(nprobe, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out, nprobe, p->stream));
+  return EM_OK;
+}
+
+// em_detector_sums with its per-tile partials in the device's fixed scratch:
+// no allocation.  The detector pass writes every partial it reduces, so the
+// scratch needs no clearing; calls that overlap on one device share it.
+em_status em_detector_sums_noalloc(em_plan p, const void* Psi, int nprobe, int ndet, const double* inner,
+                                   const double* outer, float w, int accumulate, float* out) {
+  if (!p || !Psi || !out || nprobe <= 0 || ndet <= 0 || ndet > kMaxDet || !inner || !outer)
+    return EM_ERR_INVALID;
+  if (!p->scratch) return p->fail(EM_ERR_HIP, "no device scratch");
+  int dev = -1;
+  EM_HIP(p, hipGetDevice(&dev));
+  if (dev != p->device) return p->fail(EM_ERR_INVALID, "the plan's device is not current");
+  const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
+  if ((size_t)nprobe * ndet * ntiles > kNoAllocScratchFloats)
+    return p->fail(EM_ERR_INVALID, "nprobe * ndet * (nx / 8) exceeds the fixed scratch of " +
+                                       std::to_string(kNoAllocScratchFloats) + " partial sums");
+  ColArgs c = col_args(p, reinterpret_cast<const float2*>(Psi), nullptr, nprobe);
+  c.ndet = ndet;
+  for (int d = 0; d < ndet; ++d) {
+    c.dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+    c.dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
+  }
+  c.dpart = p->scratch;
+  EM_TRY(col(p, COL_DET, c));
+  EM_HIP(p, launch_det_reduce(nprobe, ndet, ntiles, p->scratch, w, accumulate, out, nprobe, p->stream));
   return EM_OK;
 }
 

@@ -275,6 +275,17 @@ em_status em_multislice_adjoint(em_plan plan, const float* V_dev, const void* t_
 em_status em_detector_sums(em_plan plan, const void* Psi_dev, int nprobe, int ndet,
                            const double* inner_mrad, const double* outer_mrad, float w,
                            int accumulate, float* out_dev);
+/* em_detector_sums without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The detector pass's
+ * per-tile partials go to a fixed 1 MiB device array (262144 floats) that
+ * each GPU holds until the process ends, so nprobe * ndet * (nx / 8) must not
+ * exceed 262144 (EM_ERR_INVALID otherwise).  Same results as
+ * em_detector_sums.  Calls that overlap on one GPU share the array and race;
+ * the plan's device must be current. */
+em_status em_detector_sums_noalloc(em_plan plan, const void* Psi_dev, int nprobe, int ndet,
+                                   const double* inner_mrad, const double* outer_mrad, float w,
+                                   int accumulate, float* out_dev);
 /* Coherent objective-lens image (Lens_Modulation::CTF_TEM,
  * lens_modulation.cuh:78-89): I = |IFFT[Psi exp(i chi) A]|^2, or re/im when
  * re_dev/im_dev are given (I_dev NULL); accumulate adds w * value. */
