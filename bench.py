@@ -34,10 +34,13 @@ SCAN = 64
 U_RMS = 0.08
 SEED = 1234
 HBM_PEAK = 8.0e12
-# HBM traffic per probe of one launch of the fused passes at 1024^2, groups of
-# 37 probes, band skip 1e-9, row_blocks 768 / col_blocks 4096 (the autotuner's
-# usual choice): rocprofv3 --pmc FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000),
-# factors from a 1 GiB device copy; profiles/batched_stem_1024_hbm_traffic.md
+# HBM traffic of the fused passes at 1024^2 per probe and launch, from
+# rocprofv3 --pmc FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000) with factors
+# from a 1 GiB device copy (profiles/batched_stem_1024_hbm_traffic.md), and
+# the launch shapes they were measured at.  The bench reports it only when its
+# own tuning is the same: other shapes move other bytes.
+PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
+              "probe_group": 37}
 PMC_BYTES_PER_PROBE = {"column pass": (265.74 + 229.18) / 37 * 2**20, "row pass": (230.21 + 222.92) / 37 * 2**20}
 
 
@@ -48,7 +51,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--group", type=int, default=0, help="probes kept together per fused pass (0 = auto)")
-    ap.add_argument("--cpu-probes", type=int, default=16, help="probes of the CPU baseline (one thread each)")
+    ap.add_argument("--cpu-probes", type=int, default=0,
+                    help="probes of the CPU baseline, one per thread (0 = one per usable core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--npos", type=int, default=0, help="scan positions per step (0 = the full 64x64 raster)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -56,7 +60,8 @@ def parse():
                     help="row-pass workgroup target instead of the default (with --no-autotune)")
     ap.add_argument("--col-blocks", type=int, default=0,
                     help="column-pass workgroup target instead of the default (with --no-autotune)")
-    ap.add_argument("--plain-probes", type=int, default=4)
+    ap.add_argument("--plain-probes", type=int, default=32,
+                    help="probes of the one-probe-per-launch side of batched_vs_plain (each grid)")
     ap.add_argument("--no-band-skip", action="store_true",
                     help="compute every propagator column (no band-limit skipping)")
     ap.add_argument("--only-stem", action="store_true",
@@ -71,10 +76,24 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
+def cpu_cores():
+    """(threads the CPU baseline uses, what nproc reports).  A one-GPU box
+    gives its job a 16-core share whatever the machine's count is; the
+    baseline runs one probe per core of that share (or of the affinity mask,
+    if smaller)."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    return max(1, min(16, usable)), nproc
+
+
+def cpu_baseline(at, xs, ys, ngpu_sums, n_probe, threads):
     """FP64 host oracle on a bounded sample of the same workload: n_probe probes
     of phonon configuration 1 through all 80 slices (the slice loop is timed),
-    OpenMP over probes.  Also returns the GPU result's error against it."""
+    one OpenMP thread per probe.  Also returns the GPU result's error against
+    it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import concurrent.futures as cf
     import pyoracle as O
@@ -88,7 +107,7 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
     tidx = np.searchsorted(np.array(types), Z).astype(np.int32)
     S = len(i0)
     n = N_GRID
-    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         Vo = np.stack(list(ex.map(lambda s: O.potential(n, n, at["lx"], at["ly"], i0[s], i1[s], x, y, occ, tidx,
                                                          tab, sig), range(S))))
     Po = O.propagator(n, n, at["lx"], at["ly"], DZ, True, lam)
@@ -96,7 +115,6 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
     zmid0 = 0.5 * (bounds[0] + bounds[1])
     g2min = np.array([(np.sin(DET[0] * 1e-3) / lam) ** 2])
     g2max = np.array([(np.sin(DET[1] * 1e-3) / lam) ** 2])
-    threads = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     ref, used = O.stem(n, n, at["lx"], at["ly"], lam, lens, zmid0, Po, Vo, xs[:n_probe], ys[:n_probe], g2min,
                        g2max, nthreads=threads)
@@ -112,14 +130,14 @@ def cpu_baseline(at, xs, ys, ngpu_sums, n_probe):
     return n_probe * S / dt, min(used, n_probe), rel, n_probe * S, fp32
 
 
-def plain_path(em, torch, at, xs, ys, n_probe):
+def plain_path(em, torch, at, xs, ys, n_probe, n_grid):
     """The reference's one-probe-per-launch STEM path (run_STEM_general_simu,
     scan_simulator.cuh:114-183) on this build's kernels: per probe, re-cut
     the atoms around the probe (2 A margin), slice them, rebuild every
     potential slice, build the probe, run the slices, sum the detector and
     copy the result to the host.  Returns probe-slices/s."""
     L = at["lx"]
-    plan = em.Plan(N_GRID, N_GRID, L, L, DZ, E0, max_batch=1)
+    plan = em.Plan(n_grid, n_grid, L, L, DZ, E0, max_batch=1)
     lens = {"outer_aper_mrad": ALPHA}
     done = 0
 
@@ -142,6 +160,30 @@ def plain_path(em, torch, at, xs, ys, n_probe):
     dt = time.perf_counter() - t0
     plan.close()
     return done / dt
+
+
+def batched_steps(em, torch, at0, xs, ys, n_grid, steps=2, batch=256):
+    """probe-slices/s of the batched STEM steps on another grid (default
+    launch shapes, one frozen-phonon configuration per step)."""
+    L = at0["lx"]
+    plan = em.Plan(n_grid, n_grid, L, L, DZ, E0, max_batch=batch)
+    out = torch.zeros((1, len(xs)), dtype=torch.float32, device="cuda")
+    lens = {"outer_aper_mrad": ALPHA}
+
+    def step(i):
+        at = em.specimen.vibrate(at0, U_RMS, SEED, i)
+        S = plan.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        plan.stem(lens, xs, ys, DET[0], DET[1], out=out, accumulate=True)
+        return S
+
+    step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S = sum(step(2 + i) for i in range(steps))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.close()
+    return len(xs) * S / dt
 
 
 def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
@@ -355,9 +397,10 @@ def main():
                           "tuning": plan.tuning()}), flush=True)
         return
 
-    # Dominant kernels at the real shape: one batch of the same STEM slice
+    # The slice step at the real shape: one batch of the same STEM slice
     # loop run eagerly, every interior row / column pass launch bracketed by
-    # HIP events on the plan's stream; averages over all those launches.
+    # HIP events on the plan's stream; averages over all those launches.  One
+    # slice step of a probe group is one row pass plus one column pass.
     N = N_GRID * N_GRID
     tun = plan.tuning()
     nb = min(args.batch, npos)
@@ -368,32 +411,53 @@ def main():
     plan.stem(lens, xs[:nb], ys[:nb], DET[0], DET[1], w=1.0, out=tmp, accumulate=False)
     row_ms, col_ms, n_row, n_col = plan.pass_times()
     plan.set_pass_timing(False)
-    # Algorithmic bytes per launch: one read and one write of the columns the
-    # pass must touch (band skipping leaves out whole propagator columns; the
-    # column pass skips whole tiles of them), plus the shared table (t: every
-    # pixel; P: the kept columns).
+    step_ms = row_ms + col_ms
+    # Algorithmic bytes of one slice step over a group: per probe-slice one
+    # read and one write of the wave (16 B per pixel) plus the shared potential
+    # (4 B) and propagator (8 B) per pixel amortised over the group.
+    slice_bytes = (16 * N + 12 * N / probes_per_launch) * probes_per_launch
+    achieved = slice_bytes / (step_ms * 1e-3)
+    # Each pass on its own: one read and one write of the columns it must touch
+    # (band skipping leaves out whole propagator columns; the column pass skips
+    # whole tiles of them), plus its table (t: every pixel; P: kept columns).
     lo, hi = band
     skip_row = hi - lo + 1 if hi >= lo else 0
     tile = tun["col_tile"] or 8
     skip_col = tile * max(0, (hi + 1) // tile - -(-lo // tile)) if hi >= lo else 0
     row_bytes = 16 * N_GRID * (N_GRID - skip_row) * probes_per_launch + 8 * N
     col_bytes = 16 * N_GRID * (N_GRID - skip_col) * probes_per_launch + 8 * N_GRID * (N_GRID - skip_col)
-    dom = ("column pass", col_ms, col_bytes) if col_ms >= row_ms else ("row pass", row_ms, row_bytes)
-    achieved = dom[2] / (dom[1] * 1e-3)
-    traffic = PMC_BYTES_PER_PROBE[dom[0]] * probes_per_launch
+    shapes = {k: tun[k] for k in ("row_blocks", "col_blocks", "col_tile", "col_variant", "row_variant")}
+    shapes["probe_group"] = G
+    profiled = all(PMC_SHAPES[k] == v for k, v in shapes.items())
+    pmc = {k: v * probes_per_launch for k, v in PMC_BYTES_PER_PROBE.items()} if profiled else None
     copy_bw, inplace_bw = copy_bandwidth(torch)
 
-    plain_ps = plain_path(em, torch, em.specimen.vibrate(at0, U_RMS, SEED, 1), xs, ys, args.plain_probes)
+    # batched / plain on both grids: the plain side one probe per launch on a
+    # seeded sample of raster positions; the batched side the full raster
+    at1 = em.specimen.vibrate(at0, U_RMS, SEED, 1)
+    rng = np.random.default_rng(SEED)
+    sample = rng.choice(len(xs), args.plain_probes, replace=False)
+    plain_ps = plain_path(em, torch, at1, xs[sample], ys[sample], args.plain_probes, N_GRID)
+    xs32, ys32 = em.specimen.raster(32, 32, L, L)
+    sample32 = rng.choice(len(xs32), args.plain_probes, replace=False)
+    plain512 = plain_path(em, torch, at0, xs32[sample32], ys32[sample32], args.plain_probes, 512)
+    batched512 = batched_steps(em, torch, at0, xs32, ys32, 512)
     t_fwd, t_fwd_adj = adjoint_ratio(em, torch)
     rag_ps, rag_work = ragged_mix(em, torch)
 
     cpu = None
     if not args.no_cpu:
-        at1 = em.specimen.vibrate(at0, U_RMS, SEED, 1)
-        v, used, rel, units, fp32 = cpu_baseline(at1, xs, ys, sums, args.cpu_probes)
+        threads, nproc = cpu_cores()
+        n_cpu = args.cpu_probes or threads
+        v, used, rel, units, fp32 = cpu_baseline(at1, xs, ys, sums, n_cpu, threads)
+        full_s = args.steps * npos * S / v
         cpu = {"value": v, "unit": "probe-slices/s", "cores": used, "kind": "port",
-               "sample": f"{args.cpu_probes} probes x {S} slices of phonon configuration 1 "
-                         f"({units} probe-slices, FP64 host oracle, OpenMP over probes)",
+               "sample": f"{n_cpu} probes x {S} slices of phonon configuration 1 ({units} probe-slices, FP64 "
+                         f"host oracle, one OpenMP thread per probe on {threads} threads; nproc {nproc}, the "
+                         f"one-GPU box's share 16)",
+               "extrapolated_timed_steps_s": full_s,
+               "extrapolation": f"{args.steps} steps x {npos} probes x {S} slices at this rate: {full_s:.0f} s "
+                                f"against the GPU's {dt:.2f} s",
                "gpu_rel_l2_vs_oracle": rel,
                "fp32_variant": fp32}
 
@@ -416,18 +480,30 @@ def main():
                    "grid": N_GRID, "slices": S, "probes_per_step": npos, "batch": args.batch,
                    "probe_group": G, "tuning": tun, "autotune_ms_per_probe_slice": tune_ms,
                    "phonon_u_rms": U_RMS, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "launch_ms": dom[1], "launches_timed": n_col if dom[0] == "column pass" else n_row,
-                     "algorithmic_bytes_per_launch": dom[2], "probes_per_launch": probes_per_launch,
-                     "row_pass_ms": row_ms, "col_pass_ms": col_ms,
-                     "row_pass_frac": row_bytes / (row_ms * 1e-3) / HBM_PEAK,
+        # the dominant kernel is the slice step (row + column pass of one probe
+        # group): algorithmic bytes 16 N + 12 N / G per probe-slice over the
+        # event-timed launch pair
+        "roofline": {"bound": "hbm", "kernel": "slice step (row + column pass)", "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (pmc["row pass"] + pmc["column pass"]) if pmc else None,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE of both passes (copy-kernel "
+                                        "calibrated), per probe at the profiled launch shapes, scaled to "
+                                        "probes_per_launch") if pmc else
+                                       f"null: launch shapes {shapes} differ from the profiled {PMC_SHAPES}",
+                     "launch_ms": step_ms, "launches_timed": min(n_row, n_col),
+                     "algorithmic_bytes_per_launch": slice_bytes, "probes_per_launch": probes_per_launch,
+                     "probe_slices_per_s_from_launches": probes_per_launch / (step_ms * 1e-3),
                      "copy_kernel_GBps": copy_bw / 1e9, "frac_of_copy": achieved / copy_bw,
                      "in_place_kernel_GBps": inplace_bw / 1e9, "frac_of_in_place": achieved / inplace_bw,
-                     "traffic_source": "rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch (copy-kernel calibrated), "
-                                       "per probe, scaled to probes_per_launch"},
-        # whole slice step against one read + write of the wave per probe-slice
-        # (16 B per pixel) plus the shared potential and propagator tables
+                     "passes": {
+                         "row": {"launch_ms": row_ms, "launches_timed": n_row, "algorithmic_bytes": row_bytes,
+                                 "frac": row_bytes / (row_ms * 1e-3) / HBM_PEAK,
+                                 "traffic": pmc["row pass"] if pmc else None},
+                         "column": {"launch_ms": col_ms, "launches_timed": n_col, "algorithmic_bytes": col_bytes,
+                                    "frac": col_bytes / (col_ms * 1e-3) / HBM_PEAK,
+                                    "traffic": pmc["column pass"] if pmc else None}}},
+        # the whole timed job (probe builds, detector passes, batch boundaries
+        # and host work included) against the same bytes per probe-slice
         "slice_roofline_frac": ps * (16 * N + 12 * N / G) / HBM_PEAK,
         # arithmetic, reported only (HBM bounds the path): two 2D FFT pairs'
         # worth of radix-2 flops per probe-slice, 2 * 5 N log2 N
@@ -436,9 +512,15 @@ def main():
                   "fp32_peak_TFLOPs": 157.3},  # published dense FP32 rate
         "band_skip": {"threshold": 0.0 if args.no_band_skip else 1e-9, "skipped_columns": band[1] - band[0] + 1 if band[1] >= band[0] else 0,
                       "of": N_GRID},
-        "batched_vs_plain": {"ratio": ps / plain_ps, "plain_probe_slices_per_s": plain_ps,
-                             "plain_path": "one probe per launch, atoms re-cut and potentials rebuilt per probe "
-                                           f"(general scanner), {args.plain_probes} probes"},
+        "batched_vs_plain": {
+            "ratio": ps / plain_ps,
+            "1024": {"ratio": ps / plain_ps, "batched_probe_slices_per_s": ps, "plain_probe_slices_per_s": plain_ps,
+                     "plain_probes": args.plain_probes},
+            "512": {"ratio": batched512 / plain512, "batched_probe_slices_per_s": batched512,
+                    "plain_probe_slices_per_s": plain512, "plain_probes": args.plain_probes,
+                    "batched": "32x32 raster (1024 probes) x 80 slices, 2 phonon steps, default launch shapes"},
+            "plain_path": "one probe per launch, atoms re-cut and potentials rebuilt per probe (general scanner), "
+                          "seeded raster sample"},
         "ragged_mix": {"probe_slices_per_s": rag_ps, "probe_slices": rag_work,
                        "workload": "8 seeded requests on 512^2/768^2/1024^2/2048^2 grids, Si 50-400 A, "
                                    "200/300 kV, 10-30 mrad, detector sums every 10 slices, one graph launch; "
