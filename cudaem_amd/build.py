@@ -22,10 +22,16 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 
+# The passes prune line slots that hold known zeros (band-limit skip): the
+# kernels' file may drop "x + 0", which differs from x only in the sign of a
+# zero.
+EXTRA = {"em_kernels.hip": ["-fno-signed-zeros"]}
+
+
 def _compile(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
-    cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", path, "-o", out]
+    cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ["-x", "hip", "-c", path, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

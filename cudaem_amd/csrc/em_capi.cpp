@@ -62,7 +62,7 @@ struct Graph {
 // detector bands, launch shapes, band skip and the device buffers it reads
 // and writes.  Zero-filled before use, so it compares bytewise.
 struct StemGraphKey {
-  int nb, S, ndet, skip_lo, skip_hi;
+  int nb, S, ndet, skip_lo, skip_hi, skip_lo_y, skip_hi_y;
   em_tuning tune;
   double dg2min[kMaxDet], dg2max[kMaxDet];
   const void* buf[10];
@@ -151,6 +151,10 @@ struct em_plan_s {
   // are below band_skip (relative) for every ky; skip_lo > skip_hi: none
   double band_skip = 1e-9;
   int skip_lo = 1, skip_hi = 0;
+  // the same range along ky (rows of a column whose propagator is below
+  // band_skip whatever kx is), and the line slots the pruned passes may drop
+  int skip_lo_y = 1, skip_hi_y = 0;
+  int zk_x = 0, zk_y = 0;
   std::string err;
   // spline settings and element overrides
   int nR = 128;
@@ -267,20 +271,29 @@ int col_pb(const em_plan p, int B) {
 // changes the result by less than band_skip relative -- under FP32 resolution
 // for band_skip <= 1e-9.  They form the range |kx_s| >= K of the unshifted
 // index, i.e. kx in [K, nx - K].
+// The same along ky: the factor is largest at kx = 0, so rows |ky_s| >= K_y
+// of every column are below band_skip.  The pruned passes drop the line
+// slots wholly inside these ranges.
 void compute_band_skip(em_plan p) {
-  p->skip_lo = 1;
-  p->skip_hi = 0;
+  p->skip_lo = p->skip_lo_y = 1;
+  p->skip_hi = p->skip_hi_y = 0;
+  p->zk_x = p->zk_y = 0;
   if (!p->grid.bwl || !(p->band_skip > 0) || std::fabs(p->grid.dz) <= 1e-5) return;
-  const int nx = p->grid.nx;
-  for (int k = 1; k <= nx / 2; ++k) {
-    const double gx = k * p->gdev.dgx;
-    const double f = 1.0 / (1.0 + std::exp(p->gdev.alpha * (gx * gx - p->gdev.gl2_max)));
-    if (f <= p->band_skip) {
-      p->skip_lo = k;
-      p->skip_hi = nx - k;
-      return;
+  auto range = [&](int n, double dg, int* lo, int* hi) {
+    for (int k = 1; k <= n / 2; ++k) {
+      const double g = k * dg;
+      const double f = 1.0 / (1.0 + std::exp(p->gdev.alpha * (g * g - p->gdev.gl2_max)));
+      if (f <= p->band_skip) {
+        *lo = k;
+        *hi = n - k;
+        return;
+      }
     }
-  }
+  };
+  range(p->grid.nx, p->gdev.dgx, &p->skip_lo, &p->skip_hi);
+  range(p->grid.ny, p->gdev.dgy, &p->skip_lo_y, &p->skip_hi_y);
+  p->zk_x = pruned_slots(p->grid.nx, p->skip_lo, p->skip_hi);
+  p->zk_y = pruned_slots(p->grid.ny, p->skip_lo_y, p->skip_hi_y);
 }
 
 RowArgs row_args(em_plan p, const float2* in, float2* out, int B) {
@@ -307,6 +320,7 @@ RowArgs row_args_skip(em_plan p, const float2* in, float2* out, int B, bool skip
   RowArgs a = row_args(p, in, out, B);
   a.skip_in = skip_in ? 1 : 0;
   a.skip_out = skip_out ? 1 : 0;
+  a.zk = (skip_in && skip_out) ? p->zk_x : 0;
   return a;
 }
 
@@ -335,6 +349,7 @@ ColArgs col_args_skip(em_plan p, const float2* in, float2* out, int B) {
   ColArgs a = col_args(p, in, out, B);
   a.skip_lo = p->skip_lo;
   a.skip_hi = p->skip_hi;
+  a.zk = p->zk_y;
   return a;
 }
 
@@ -1022,6 +1037,7 @@ em_status em_multislice_forward_multi(em_plan p, const float* const* V, const in
       ra.chunks = dchunks + l.first;
       ra.skip_in = l.skip_in;
       ra.skip_out = l.skip_out;
+      ra.zk = (l.skip_in && l.skip_out) ? p->zk_x : 0;
       EM_TRY(row(p, l.mode, ra));
     } else {
       ColArgs cc = col_args(p, nullptr, nullptr, l.count);
@@ -1031,6 +1047,7 @@ em_status em_multislice_forward_multi(em_plan p, const float* const* V, const in
       cc.variant = COL_VARIANT_BLOCK;
       cc.skip_lo = p->skip_lo;
       cc.skip_hi = p->skip_hi;
+      cc.zk = p->zk_y;
       EM_TRY(col(p, l.mode, cc));
     }
   }
@@ -1435,6 +1452,8 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
         key.ndet = ndet;
         key.skip_lo = p->skip_lo;
         key.skip_hi = p->skip_hi;
+        key.skip_lo_y = p->skip_lo_y;
+        key.skip_hi_y = p->skip_hi_y;
         key.tune = p->tune;
         for (int d = 0; d < ndet; ++d) {
           key.dg2min[d] = dg2min[d];
@@ -1549,7 +1568,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
     for (; c < cls.size(); ++c) {
       em_plan q = plans[cls[c][0]];
       if (q->grid.nx == p->grid.nx && q->grid.ny == p->grid.ny && q->skip_lo == p->skip_lo &&
-          q->skip_hi == p->skip_hi)
+          q->skip_hi == p->skip_hi && q->skip_lo_y == p->skip_lo_y && q->skip_hi_y == p->skip_hi_y)
         break;
     }
     if (c == cls.size()) cls.emplace_back();
@@ -1682,6 +1701,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
         ra.chunks = dchunks + op.first;
         ra.skip_in = op.skip_in;
         ra.skip_out = op.skip_out;
+        ra.zk = (op.skip_in && op.skip_out) ? pc->zk_x : 0;
         st = pc->hip(launch_row(op.mode, ra, br), "row pass");
       } else if (op.kind == OP_COL) {
         ColArgs cc = col_args(pc, nullptr, nullptr, op.count);
@@ -1691,6 +1711,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
         cc.variant = COL_VARIANT_BLOCK;
         cc.skip_lo = pc->skip_lo;
         cc.skip_hi = pc->skip_hi;
+        cc.zk = pc->zk_y;
         st = pc->hip(launch_col(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS, cc, br),
                      "column pass");
       } else {

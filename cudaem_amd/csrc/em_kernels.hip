@@ -116,6 +116,43 @@ EM_FFT_CFG(2048, 128, 16, 16, 8)
 
 constexpr int kRowThreads = 256;
 
+// Band-limit pruning.  A line of N points held by T lanes, E = N / T values
+// each, puts element tau + T e in slot e.  The band-limit skip range
+// [skip_lo, skip_hi] is symmetric about N / 2, so the slots wholly inside it
+// are E/2 - ZK <= e < E/2 + ZK for some ZK.  In those slots the wave is zero
+// on input (the propagator was below the skip threshold there) and not needed
+// on output, so a pruned pass loads, multiplies and stores nothing there and
+// the transforms' first stage adds known zeros (folded at compile time).
+// PrunedSlots: the ZK the pruned kernels of a line length are built for (the
+// skip ranges of the 2/3 band limit hold about a quarter of the line);
+// pruned_slots: the ZK a skip range allows.
+template <int N>
+struct PrunedSlots {
+  static constexpr int value = (N / FftCfg<N>::T) / 8;
+};
+template <int E, int ZK>
+__device__ __forceinline__ constexpr bool zero_slot(int e) {
+  return ZK > 0 && e >= E / 2 - ZK && e < E / 2 + ZK;
+}
+int pruned_slots(int n, int lo, int hi) {
+  int T = 0;
+  switch (n) {
+#define EM_T(N_)         \
+  case N_:               \
+    T = FftCfg<N_>::T;   \
+    break;
+    EM_T(64) EM_T(128) EM_T(256) EM_T(512) EM_T(768) EM_T(1024) EM_T(2048)
+#undef EM_T
+    default:
+      return 0;
+  }
+  if (lo > hi) return 0;
+  const int E = n / T;
+  int z = 0;
+  while (z < E / 2 && T * (E / 2 - (z + 1)) >= lo && T * (E / 2 + (z + 1)) - 1 <= hi) ++z;
+  return z;
+}
+
 // Columns per column-pass workgroup: the default for a column length N, and
 // the alternative the autotuner may pick for the per-slice modes. Eight
 // columns (a 64 KiB + pad LDS line buffer) let two workgroups share a CU, so
@@ -160,7 +197,8 @@ struct RowOcc {
   static constexpr int waves = (MODE & ROW_T_STREAM) ? 4 : 1;
 };
 
-template <int NX, unsigned MODE>
+// ZK > 0: the pruned form of a slice-loop pass (skip_in and skip_out set).
+template <int NX, unsigned MODE, int ZK = 0>
 __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(RowOcc<MODE>::waves)))
 void row_pass_kernel(RowArgs a) {
   constexpr int T = FftCfg<NX>::T;
@@ -229,8 +267,12 @@ void row_pass_kernel(RowArgs a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kx = tau + T * e;
-      const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
-      dst[e] = skip ? make_float2(0.f, 0.f) : in[rbase + kx];
+      if (zero_slot<E, ZK>(e)) {
+        dst[e] = make_float2(0.f, 0.f);
+      } else {
+        const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
+        dst[e] = skip ? make_float2(0.f, 0.f) : in[rbase + kx];
+      }
     }
   };
   if constexpr (PREFETCH) {
@@ -295,7 +337,7 @@ void row_pass_kernel(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int kx = tau + T * e;
-        if (!(a.skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) out[base + kx] = v[e];
+        if (!zero_slot<E, ZK>(e) && !(a.skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) out[base + kx] = v[e];
       }
     }
   }
@@ -330,7 +372,9 @@ struct ColLateP {
 
 // At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
 // 8-column tile then share a CU, one loading while the other computes.
-template <int NY, int COLS_, unsigned MODE>
+// ZK > 0: the pruned form of a propagator pass (ky slots wholly inside the
+// band-limit skip: their propagator is below the skip threshold).
+template <int NY, int COLS_, unsigned MODE, int ZK = 0>
 __global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 void col_pass_kernel(ColArgs a) {
   using C = ColCfg<NY, COLS_>;
@@ -385,8 +429,12 @@ void col_pass_kernel(ColArgs a) {
   if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0 && !LATE_P) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      p[e] = Ptab[(long long)(tau + T * e) * nx + x];
-      if constexpr ((MODE & COL_CONJ_P) != 0) p[e].y = -p[e].y;
+      if (zero_slot<E, ZK>(e)) {
+        p[e] = make_float2(0.f, 0.f);
+      } else {
+        p[e] = Ptab[(long long)(tau + T * e) * nx + x];
+        if constexpr ((MODE & COL_CONJ_P) != 0) p[e].y = -p[e].y;
+      }
     }
   } else if constexpr ((MODE & (COL_LOAD_PROBE | COL_LOAD_MUL_A)) != 0) {
 #pragma unroll
@@ -454,13 +502,17 @@ void col_pass_kernel(ColArgs a) {
     if constexpr (LATE_P) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        float2 pe = Ptab[(long long)(tau + T * e) * nx + x];
-        if constexpr ((MODE & COL_CONJ_P) != 0) pe.y = -pe.y;
-        v[e] = cmul(v[e], pe);
+        if (zero_slot<E, ZK>(e)) {
+          v[e] = make_float2(0.f, 0.f);
+        } else {
+          float2 pe = Ptab[(long long)(tau + T * e) * nx + x];
+          if constexpr ((MODE & COL_CONJ_P) != 0) pe.y = -pe.y;
+          v[e] = cmul(v[e], pe);
+        }
       }
     } else if constexpr ((MODE & (COL_MUL_P | COL_CONJ_P)) != 0) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
+      for (int e = 0; e < E; ++e) v[e] = zero_slot<E, ZK>(e) ? make_float2(0.f, 0.f) : cmul(v[e], p[e]);
     }
     if constexpr ((MODE & COL_MUL_FILTER) != 0) {
 #pragma unroll
@@ -615,6 +667,19 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
   X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)
 
+// slice-loop modes that also exist in pruned form (interior slices)
+#define EM_ROW_PRUNED_MODES(X)                                                  \
+  X(ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT)                                    \
+  X(ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT)                                    \
+  X(ROW_PRE_IFFT | ROW_MUL_V | ROW_POST_FFT | ROW_CHUNKS)
+
+#define EM_COL_PRUNED_MODES(X)                                                  \
+  X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE)                        \
+  X(COL_PRE_FFT | COL_CONJ_P | COL_POST_IFFT | COL_STORE)
+
+#define EM_COL_PRUNED_CHUNK_MODES(X)                                            \
+  X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS)
+
 #define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
 bool fft_size_supported(int n) {
@@ -641,6 +706,18 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   constexpr int LINES = kRowThreads / FftCfg<NX>::T;
   const dim3 grid(a.ny / LINES, (a.nprobe + a.pb - 1) / a.pb);
   constexpr size_t lds = row_lds<NX>();
+  constexpr int ZK = PrunedSlots<NX>::value;
+  if (ZK > 0 && a.zk >= ZK && a.skip_in && a.skip_out) {
+    switch (mode) {
+#define EM_ROW_PRUNED_CASE(M) \
+  case (M):                   \
+    return launch_pass(row_pass_kernel<NX, (M), ZK>, grid, dim3(kRowThreads), lds, s, a);
+      EM_ROW_PRUNED_MODES(EM_ROW_PRUNED_CASE)
+#undef EM_ROW_PRUNED_CASE
+      default:
+        break;
+    }
+  }
   switch (mode) {
 #define EM_ROW_CASE(M)                                                            \
   case (M):                                                                       \
@@ -665,13 +742,13 @@ static int skipped_tiles(unsigned mode, ColArgs& a) {
   return hi - lo + 1;
 }
 
-template <int NY, int COLS, unsigned M>
+template <int NY, int COLS, unsigned M, int ZK = 0>
 static hipError_t col_launch_one(const ColArgs& args, hipStream_t s) {
   using C = ColCfg<NY, COLS>;
   ColArgs a = args;
   const int skipped = skipped_tiles<COLS>(M, a);
   const dim3 grid(a.nx / C::COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
-  return launch_pass(col_pass_kernel<NY, COLS, M>, grid, dim3(C::THREADS), C::LDS, s, a);
+  return launch_pass(col_pass_kernel<NY, COLS, M, ZK>, grid, dim3(C::THREADS), C::LDS, s, a);
 }
 
 template <int NY>
@@ -691,6 +768,25 @@ static hipError_t col_attrs(void) {
                             (int)ColCfg<NY, AltCols<NY>::value>::LDS);
   EM_COL_HOT_MODES(EM_COL_ATTR_ALT)
 #undef EM_COL_ATTR_ALT
+  constexpr int ZK = PrunedSlots<NY>::value;
+  if constexpr (ZK > 0) {
+#define EM_COL_ATTR_PR(M)                                                                        \
+    if (e == hipSuccess)                                                                         \
+      e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, (M), ZK>,   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ColCfg<NY>::LDS);  \
+    if (e == hipSuccess)                                                                         \
+      e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, AltCols<NY>::value, (M), ZK>,       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,                        \
+                              (int)ColCfg<NY, AltCols<NY>::value>::LDS);
+    EM_COL_PRUNED_MODES(EM_COL_ATTR_PR)
+#undef EM_COL_ATTR_PR
+#define EM_COL_ATTR_PRC(M)                                                                       \
+    if (e == hipSuccess)                                                                         \
+      e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, (M), ZK>,   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ColCfg<NY>::LDS);
+    EM_COL_PRUNED_CHUNK_MODES(EM_COL_ATTR_PRC)
+#undef EM_COL_ATTR_PRC
+  }
   if constexpr (FftCfg<NY>::T == 64) {
 #define EM_COL_ATTR_T(M)                                                                 \
     if (e == hipSuccess)                                                                 \
@@ -753,6 +849,25 @@ static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
     return a.cols == 8 ? col_launch_t<NY, 8, (M)>(a, s) : col_launch_t<NY, 16, (M)>(a, s);
       EM_COL_HOT_MODES(EM_COL_T_CASE)
 #undef EM_COL_T_CASE
+      default:
+        break;
+    }
+  }
+  constexpr int ZK = PrunedSlots<NY>::value;
+  if (ZK > 0 && a.zk >= ZK) {
+    const bool alt = a.cols == AltCols<NY>::value;
+    switch (mode) {
+#define EM_COL_PR_CASE(M)                                                                   \
+  case (M):                                                                                 \
+    return alt ? col_launch_one<NY, AltCols<NY>::value, (M), ZK>(a, s)                      \
+               : col_launch_one<NY, DefaultCols<NY>::value, (M), ZK>(a, s);
+      EM_COL_PRUNED_MODES(EM_COL_PR_CASE)
+#undef EM_COL_PR_CASE
+#define EM_COL_PRC_CASE(M) \
+  case (M):                \
+    return col_launch_one<NY, DefaultCols<NY>::value, (M), ZK>(a, s);
+      EM_COL_PRUNED_CHUNK_MODES(EM_COL_PRC_CASE)
+#undef EM_COL_PRC_CASE
       default:
         break;
     }

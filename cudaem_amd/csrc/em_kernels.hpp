@@ -80,6 +80,9 @@ struct RowArgs {
   // skip_in: those input columns are taken as zero and not read;
   // skip_out: those output columns are not written.
   int skip_lo, skip_hi, skip_in, skip_out;
+  // slots per line end wholly inside [skip_lo, skip_hi] (pruned_slots): with
+  // skip_in and skip_out set, the slice-loop modes then run pruned
+  int zk;
   const PassChunk* chunks;  // ROW_CHUNKS: one entry per workgroup row
 };
 
@@ -125,6 +128,9 @@ struct ColArgs {
   // neither read nor written (set from skip_lo/hi by the launcher)
   int skip_lo, skip_hi;
   int tile_lo, tile_hi;
+  // propagator modes: ky slots wholly inside the band-limit skip range of
+  // the column (pruned_slots of ny and that range); > 0 runs them pruned
+  int zk;
   const PassChunk* chunks;  // COL_CHUNKS: one entry per workgroup row
 };
 
@@ -151,6 +157,10 @@ int col_alt_tile(int ny);  // the alternative tile of the per-slice column modes
 // whether the transposed-exchange column pass exists for this column length
 bool col_transposed_available(int ny);
 int col_ntiles(int nx, int ny);
+// Slots per line end that a band-limit skip range [lo, hi] of a line of n
+// points covers wholly (the pruned passes need at least the slots they are
+// built for).
+int pruned_slots(int n, int lo, int hi);
 
 // --- other kernels ------------------------------------------------------------
 // fine: null, or one float4 (c0, c1, c2, c3) per spline node of every atom
