@@ -146,7 +146,7 @@ struct em_plan_s {
   Constants k{};
   double vr_factor = 0, gx0 = 0, gy0 = 0;
   GridDev gdev{};
-  em_tuning tune{4096, 8192, 0, 0, 0, 0};
+  em_tuning tune{4096, 8192, 0, 0, 0, 0, 0};
   // band-limit skipping: columns kx in [skip_lo, skip_hi] of the propagator
   // are below band_skip (relative) for every ky; skip_lo > skip_hi: none
   double band_skip = 1e-9;
@@ -186,6 +186,8 @@ struct em_plan_s {
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
+  DevBuf pk_ctr, pk_err;  // persistent slice loop: work queues and barrier count, error flag
+  int pk_grid = 0;        // its workgroups (0: not yet queried)
   DevBuf chunks;   // chunk tables of the ragged path
   DevBuf mchunks;  // chunk tables of em_multislice_forward_multi
   int vstore_slices = -1;
@@ -428,10 +430,83 @@ em_status thickness_loop(em_plan p, int S, float2* wv, int nprobe, double inner,
   return EM_OK;
 }
 
+// Whether the interior of a slice loop on precomputed transmissions runs as
+// one persistent launch (tuning persistent, and a grid and skip range its
+// pruned items support).
+bool use_persistent(em_plan p, bool on_t, int S) {
+  return p->tune.persistent && on_t && S >= 2 && p->tune.row_variant == 0 &&
+         persist_supported(p->grid.nx, p->grid.ny, p->zk_x, p->zk_y);
+}
+
+// The propagator passes of slices 0..S-2 and the row passes of slices
+// 1..S-2 of nb probes at wave, in one persistent launch: the caller runs
+// slice 0's row pass before and slice S-1's row pass and last column pass
+// after.  Work items follow the launch-shape targets: row_blocks row items of
+// 8 rows, col_blocks column items of 8 columns.
+em_status persistent_interior(em_plan p, const float2* t, int S, float2* wave, int nb) {
+  if (p->pk_grid <= 0) EM_HIP(p, persist_grid(p->grid.nx, &p->pk_grid));
+  const int n = p->grid.nx;
+  const int ctr = persist_counters(S);
+  EM_HIP(p, p->pk_ctr.ensure(sizeof(float2) * (size_t)((ctr + 1) / 2)));
+  EM_HIP(p, launch_fill(p->pk_ctr.as<float2>(), (ctr + 1) / 2, make_float2(0.f, 0.f), p->stream));
+  PersistArgs a{};
+  a.wave = wave;
+  a.pstride = p->N();
+  a.n = n;
+  a.nprobe = nb;
+  a.S = S;
+  a.t = t;
+  a.P = p->P.as<float2>();
+  a.twx = p->tw_x.as<float2>();
+  a.twy = p->tw_y.as<float2>();
+  a.skip_lo = p->skip_lo;
+  a.skip_hi = p->skip_hi;
+  a.tile_lo = (p->skip_lo + 7) / 8;
+  a.tile_hi = (p->skip_hi + 1) / 8 - 1;
+  a.zk_x = p->zk_x;
+  a.zk_y = p->zk_y;
+  const int row_units = n / 8;
+  const int kept = n / 8 - (a.tile_lo <= a.tile_hi ? a.tile_hi - a.tile_lo + 1 : 0);
+  const int rch = std::max(1, std::min(nb, (p->tune.row_blocks + row_units - 1) / row_units));
+  const int cch = std::max(1, std::min(nb, (p->tune.col_blocks + kept - 1) / kept));
+  a.row_pb = (nb + rch - 1) / rch;
+  a.col_pb = (nb + cch - 1) / cch;
+  a.ctr = reinterpret_cast<int*>(p->pk_ctr.p);
+  a.err = reinterpret_cast<int*>(p->pk_err.p);
+  a.spin_limit = 50000000;  // 0.5 s of the 100 MHz wall clock
+  return p->hip(launch_persist(a, p->pk_grid, p->stream), "persistent slice loop");
+}
+
+// Before a call's persistent launches: clear the error flag.
+em_status persistent_begin(em_plan p) {
+  EM_HIP(p, p->pk_err.ensure(sizeof(float2)));
+  EM_HIP(p, launch_fill(p->pk_err.as<float2>(), 1, make_float2(0.f, 0.f), p->stream));
+  return EM_OK;
+}
+
+// After a call's persistent launches: wait for them and report a barrier
+// that gave up.
+em_status persistent_check(em_plan p) {
+  int err = 0;
+  EM_HIP(p, hipStreamSynchronize(p->stream));
+  EM_HIP(p, hipMemcpy(&err, p->pk_err.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (err != 0) return p->fail(EM_ERR_HIP, "persistent slice loop: a grid barrier wait timed out");
+  return EM_OK;
+}
+
 // Fused forward multislice over nb probes at wave (real space in, Fourier out).
 em_status forward_group(em_plan p, const float* V, const float2* t, int S, float2* wave, int nb,
                         float2* ckpt, long long ckpt_slice_stride, int every, bool in_mixed_layout) {
+  const bool pk = !ckpt && use_persistent(p, t != nullptr && V == nullptr, S);
   for (int s = 0; s < S; ++s) {
+    if (pk && s > 0 && s < S - 1) continue;  // in the persistent launch
+    if (pk && s == 0) {
+      RowArgs r = row_args_skip(p, wave, wave, nb, in_mixed_layout, true);
+      r.t = t;
+      EM_TRY(row(p, ROW_POST_FFT | ROW_MUL_T | (in_mixed_layout ? ROW_PRE_IFFT : 0u), r));
+      EM_TRY(persistent_interior(p, t, S, wave, nb));
+      continue;
+    }
     RowArgs r = row_args_skip(p, wave, wave, nb, s > 0 || in_mixed_layout, s < S - 1);
     unsigned m = ROW_POST_FFT;
     if (s > 0 || in_mixed_layout) m |= ROW_PRE_IFFT;
@@ -618,7 +693,7 @@ em_status em_plan_get_tuning(em_plan plan, em_tuning* t) {
 
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* t) {
   if (!plan || !t || t->row_blocks <= 0 || t->col_blocks <= 0 || t->probe_group < 0 || t->col_variant < 0 ||
-      t->col_variant > 2 || t->row_variant < 0 || t->row_variant > 2)
+      t->col_variant > 2 || t->row_variant < 0 || t->row_variant > 2 || t->persistent < 0 || t->persistent > 1)
     return EM_ERR_INVALID;
   if (t->col_tile != 0 && t->col_tile != col_tile(plan->grid.ny) &&
       (t->col_tile != col_alt_tile(plan->grid.ny) || plan->grid.nx % t->col_tile != 0) &&
@@ -946,12 +1021,15 @@ em_status em_multislice_forward(em_plan p, const float* V, const void* t, int S,
   if (!p || !psi || S <= 0 || nprobe <= 0 || (!V) == (!t) || every <= 0) return EM_ERR_INVALID;
   float2* w = reinterpret_cast<float2*>(psi);
   const int G = group_size(p, nprobe);
+  const bool pk = !ckpt && use_persistent(p, t && !V, S);
+  if (pk) EM_TRY(persistent_begin(p));
   for (int g0 = 0; g0 < nprobe; g0 += G) {
     const int nb = std::min(G, nprobe - g0);
     float2* ck = ckpt ? reinterpret_cast<float2*>(ckpt) + (long long)g0 * p->N() : nullptr;
     EM_TRY(forward_group(p, V, reinterpret_cast<const float2*>(t), S, w + (long long)g0 * p->N(), nb, ck,
                          (long long)nprobe * p->N(), every, false));
   }
+  if (pk) EM_TRY(persistent_check(p));
   return EM_OK;
 }
 
@@ -1335,6 +1413,7 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
   const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
   const int G = group_size(p, nb);
   float2* wave = p->wave.as<float2>();
+  const bool pk = use_persistent(p, p->tune.row_variant != 2, S);
   for (int g0 = 0; g0 < nb; g0 += G) {
     const int ng = std::min(G, nb - g0);
     ColArgs c = col_args(p, nullptr, wave, ng);
@@ -1343,6 +1422,7 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
     c.ey = p->ey.as<float2>() + (long long)g0 * p->grid.ny;
     EM_TRY(col(p, COL_LOAD_PROBE | COL_POST_IFFT | COL_STORE, c));
     for (int s = 0; s < S; ++s) {
+      if (pk && s > 0 && s < S - 1) continue;  // in the persistent launch
       // slice 0 reads every column of the freshly built probe (the condenser
       // aperture may reach past the skipped band); later slices read the
       // output of a propagator pass, zero there
@@ -1357,6 +1437,10 @@ em_status stem_batch_body(em_plan p, int nb, int S, const double* dg2min, const 
         rm = row_t_mode(p, ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT);
       }
       EM_TRY(timed(p, 0, [&] { return row(p, rm, r); }));
+      if (pk && s == 0) {
+        EM_TRY(persistent_interior(p, p->Tstore.as<float2>(), S, wave, ng));
+        continue;
+      }
       ColArgs cc = col_args_skip(p, wave, wave, ng);
       if (s < S - 1) {
         EM_TRY(timed(p, 1, [&] { return col(p, COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE, cc); }));
@@ -1422,6 +1506,9 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
   // the private graph stream runs after the caller's stream, and the
   // caller's stream after it: no host synchronisation in the call
   const hipStream_t user = p->stream;
+  // the persistent slice loop runs eagerly (a cooperative launch), and the
+  // call waits for it to check its barriers
+  const bool pk = use_persistent(p, p->tune.row_variant != 2, S);
   EM_HIP(p, hipEventRecord(p->stem_in, user));
   EM_HIP(p, hipStreamWaitEvent(p->gstream, p->stem_in, 0));
   em_status st = EM_OK;
@@ -1436,12 +1523,13 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
     if (st == EM_OK)
       st = p->hip(launch_lens_table(p->gdev, derive_lens(*cond, p->k.lambda, zmid0), p->A.as<float2>(), p->stream),
                   "lens table");
+    if (st == EM_OK && pk) st = persistent_begin(p);
     for (int start = 0; start < npos && st == EM_OK; start += B) {
       const int nb = std::min(B, npos - start);
       if ((st = p->hip(launch_probe_ramps(p->gdev, nb, xe + start, ye + start, p->ex.as<float2>(),
                                           p->ey.as<float2>(), p->stream), "probe ramps")) != EM_OK)
         break;
-      if (p->pass_timing) {
+      if (p->pass_timing || pk) {
         // timed launches run eagerly: events between the passes need a plain stream
         if ((st = stem_batch_body(p, nb, S, dg2min, dg2max, ndet)) != EM_OK) break;
       } else {
@@ -1486,6 +1574,7 @@ em_status em_stem(em_plan p, const em_lens* cond, int npos, const double* x, con
       st = p->hip(launch_det_reduce(nb, ndet, ntiles, p->dpart.as<float>(), w, accumulate, out + start, npos, p->stream),
                   "detector reduce");
     }
+    if (st == EM_OK && pk) st = persistent_check(p);
   }
   (void)hipEventRecord(p->stem_out, p->gstream);
   (void)hipStreamWaitEvent(user, p->stem_out, 0);
@@ -1918,15 +2007,24 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   const int tiles[] = {0, col_alt_tile(p->grid.ny)};
   const int variants[] = {1, 2};
   const int row_variants[] = {0, 1, 2};
+  // the persistent slice loop where the grid supports it
+  const int persistents[] = {0, 1};
+  const int npers = persist_supported(p->grid.nx, p->grid.ny, p->zk_x, p->zk_y) ? 2 : 1;
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
-    for (int par = 0; par < 6 && st == EM_OK; ++par) {
+    for (int par = 0; par < 7 && st == EM_OK; ++par) {
       const int* cand = par == 0   ? groups
                         : par == 1 ? rowb.data()
                         : par == 2 ? colb
                         : par == 3 ? tiles
                         : par == 4 ? variants
-                                   : row_variants;
-      const int nc = par == 0 ? 9 : par == 1 ? (int)rowb.size() : par == 2 ? 7 : par == 5 ? 3 : 2;
+                        : par == 5 ? row_variants
+                                   : persistents;
+      const int nc = par == 0   ? 9
+                     : par == 1 ? (int)rowb.size()
+                     : par == 2 ? 7
+                     : par == 5 ? 3
+                     : par == 6 ? npers
+                                : 2;
       em_tuning bestt = p->tune;
       for (int i = 0; i < nc && st == EM_OK; ++i) {
         em_tuning t = p->tune;
@@ -1939,6 +2037,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
         }
         if (par == 4) t.col_variant = cand[i];
         if (par == 5) t.row_variant = cand[i];
+        if (par == 6) t.persistent = cand[i];
         if (par == 0 && cand[i] > nprobe) continue;
         const em_tuning keep = p->tune;
         p->tune = t;

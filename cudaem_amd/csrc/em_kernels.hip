@@ -617,6 +617,261 @@ __global__ __launch_bounds__(COLS * 64) void col_pass_t_kernel(ColArgs a) {
 }
 
 // --------------------------------------------------------------------------
+// Persistent slice loop of one probe group (batched STEM, scan_simulator.cuh:
+// 87-109 -> wave_function.cuh:74-87): one cooperative launch runs the row
+// pass of every slice and the propagator column pass of every slice but the
+// last, the phases separated by a grid barrier.  Its workgroups (512
+// threads, two per CU) take work items from per-XCD queues: a row item is 8
+// rows of a probe chunk (one row per wavefront), a column item 8 columns of
+// a probe chunk, so each XCD keeps to one band of rows or column tiles and
+// the transmissions and propagator it reads stay in its L2.  The barrier's
+// wait is bounded by the wall clock: past the limit a workgroup raises the
+// error flag and every workgroup leaves the kernel.
+// --------------------------------------------------------------------------
+constexpr int kPersistThreads = 512;
+constexpr int kXcds = 8;
+
+template <int N>
+struct PersistCfg {
+  static constexpr int T = FftCfg<N>::T;
+  static constexpr int E = N / T;
+  static constexpr int LINES = kPersistThreads / T;  // rows of a row item
+  static constexpr int COLS = kPersistThreads / T;   // columns of a column item
+  static constexpr int STRIDE = LineLds<N>::size + 16 / COLS;
+  static constexpr int AREA =
+      (LINES * LineLds<N>::size > COLS * STRIDE) ? LINES * LineLds<N>::size : COLS * STRIDE;
+  static constexpr int TAB = FftCfg<N>::TwTab::entries;  // row-pass twiddles in LDS
+  static constexpr size_t LDS = ((size_t)AREA + TAB) * sizeof(float2);
+  // one wavefront per line, column twiddles formed in registers
+  static constexpr bool ok = T == 64 && FftCfg<N>::Tw::in_registers && FftCfg<N>::TwTab::ok;
+};
+
+// A kernel-argument pointer the compiler must re-read at this point: keeps
+// per-thread addresses derived from it from being hoisted out of the
+// persistent loops, where they would stay live (and spill) across phases.
+// It is stated global again (address space 1) after the barrier, so accesses
+// stay global loads with a scalar base, not flat ones with a 64-bit address
+// per lane.
+template <class T>
+__device__ __forceinline__ T* persist_fresh(T* p) {
+  unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  asm volatile("" : "+s"(v));
+  using G = __attribute__((address_space(1))) std::remove_const_t<T>;
+  return (T*)reinterpret_cast<G*>(v);
+}
+
+__device__ __forceinline__ int persist_claim(int* q, int* slot) {
+  if (threadIdx.x == 0) *slot = __hip_atomic_fetch_add(q, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  // uniform across the workgroup: keep it (and what derives from it) scalar
+  const int it = __builtin_amdgcn_readfirstlane(*slot);
+  __syncthreads();
+  return it;
+}
+
+// Grid barrier k (k = 0, 1, ...): every workgroup adds one to *count and
+// waits until it reaches (k + 1) * gridDim.x.  The wait ends at the limit
+// (wall-clock ticks, 100 MHz) or when another workgroup has raised *err.
+// The workgroup's stores reach the device through one agent-scope release
+// by its first thread after the workgroup barrier (an agent-scope fence
+// writes back this XCD's L2 and invalidates it on acquire: one per
+// workgroup, not one per wavefront).
+__device__ __forceinline__ bool persist_barrier(int* count, int* err, int k, long long limit, int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int target = (k + 1) * (int)gridDim.x;
+    const long long t0 = wall_clock64();
+    // relaxed polling (an acquire load invalidates the L2 at every poll),
+    // one acquire fence once the count is reached
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        *flag = 1;
+        break;
+      }
+      if (wall_clock64() - t0 > limit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *flag = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return *flag == 0;
+}
+
+// Row item: rows [8 rb, 8 rb + 8) of probes [b0, b1): IFFT_x, x t, FFT_x.
+template <int N, int ZK>
+__device__ __forceinline__ void persist_row_item(const PersistArgs& a, const float2* t_sl, int rb, int b0, int b1,
+                                                 bool skip_in, bool skip_out, float2* smem, const float2* tab) {
+  using C = PersistCfg<N>;
+  constexpr int T = C::T, E = C::E;
+  const int line = threadIdx.x / T, tau = threadIdx.x % T;
+  const long long row = (long long)(rb * C::LINES + line) * N;
+  float2* lds = smem + line * LineLds<N>::size;
+  typename FftCfg<N>::TwTab tws;
+  tws.init(tau, tab);
+  t_sl = persist_fresh(t_sl);
+  // the same wave through two pointers the compiler cannot equate: it then
+  // forms each store address at the store instead of holding the load's
+  const float2* win = persist_fresh(a.wave);
+  float2* wout = persist_fresh(a.wave);
+  const float2* twx = persist_fresh(a.twx);
+  float2 t[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) t[e] = t_sl[row + tau + T * e];
+  for (int b = b0; b < b1; ++b) {
+    const long long base = (long long)b * a.pstride + row;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = tau + T * e;
+      if (zero_slot<E, ZK>(e)) {
+        v[e] = make_float2(0.f, 0.f);
+      } else {
+        const bool skip = skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
+        v[e] = skip ? make_float2(0.f, 0.f) : win[base + kx];
+      }
+    }
+    FftCfg<N>::template run<+1, SYNC_WAVE>(v, lds, tau, tws, twx);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], t[e]);
+    FftCfg<N>::template run<-1, SYNC_WAVE>(v, lds, tau, tws, twx);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = tau + T * e;
+      if (!zero_slot<E, ZK>(e) && !(skip_out && kx >= a.skip_lo && kx <= a.skip_hi)) wout[base + kx] = v[e];
+    }
+  }
+}
+
+// Column item: columns [8 tile, 8 tile + 8) of probes [b0, b1): FFT_y, x P,
+// IFFT_y.
+template <int N, int ZK>
+__device__ __forceinline__ void persist_col_item(const PersistArgs& a, int tile, int b0, int b1, float2* smem) {
+  using C = PersistCfg<N>;
+  constexpr int T = C::T, E = C::E;
+  const int c = threadIdx.x % C::COLS, tau = threadIdx.x / C::COLS;
+  const int x = tile * C::COLS + c;
+  float2* lds = smem + c * C::STRIDE;
+  const float2* twy = persist_fresh(a.twy);
+  const float2* P = persist_fresh(a.P);
+  const float2* win = persist_fresh(a.wave);
+  float2* wout = persist_fresh(a.wave);
+  typename FftCfg<N>::Tw tws;
+  tws.init(tau, twy);
+  float2 p[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    p[e] = zero_slot<E, ZK>(e) ? make_float2(0.f, 0.f) : P[(long long)(tau + T * e) * N + x];
+  for (int b = b0; b < b1; ++b) {
+    const long long base = (long long)b * a.pstride + x;
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = win[base + (long long)(tau + T * e) * N];
+    FftCfg<N>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, twy);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = zero_slot<E, ZK>(e) ? make_float2(0.f, 0.f) : cmul(v[e], p[e]);
+    FftCfg<N>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, twy);
+#pragma unroll
+    for (int e = 0; e < E; ++e) wout[base + (long long)(tau + T * e) * N] = v[e];
+  }
+}
+
+// Units [lo, hi) of XCD band x out of `total` (row blocks or kept tiles).
+__device__ __forceinline__ void persist_band(int total, int x, int* lo, int* hi) {
+  *lo = (int)((long long)total * x / kXcds);
+  *hi = (int)((long long)total * (x + 1) / kXcds);
+}
+
+// One workgroup per CU: both phases in one register allocation need 232
+// VGPRs (at a 128-VGPR budget, two workgroups per CU, they spill ~130).
+template <int N>
+__global__ __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void persist_kernel(PersistArgs a) {
+  using C = PersistCfg<N>;
+  extern __shared__ float2 smem[];
+  __shared__ int s_item, s_flag;
+  if (threadIdx.x == 0) s_flag = 0;
+  float2* tab = smem + C::AREA;
+  FftCfg<N>::TwTab::fill(tab, a.twx, threadIdx.x, kPersistThreads);
+  __syncthreads();
+  const int xcd = blockIdx.x % kXcds;
+  const int row_units = N / C::LINES;
+  const int tiles = N / C::COLS;
+  const int skipped = (a.tile_lo <= a.tile_hi) ? a.tile_hi - a.tile_lo + 1 : 0;
+  const int col_units = tiles - skipped;
+  const int rch = (a.nprobe + a.row_pb - 1) / a.row_pb;
+  const int cch = (a.nprobe + a.col_pb - 1) / a.col_pb;
+  int* count = a.ctr + 2 * a.S * kXcds;
+  int* err = a.err;
+  constexpr int ZK = PrunedSlots<N>::value;
+  int k = 0;
+  // column pass of slice s, then the row pass of slice s + 1 (interior:
+  // reads and writes only the kept columns); the host runs slice 0's and the
+  // last slice's row passes
+  for (int s = 0; s < a.S - 1; ++s) {
+    int lo, hi;
+    persist_band(col_units, xcd, &lo, &hi);
+    for (;;) {
+      const int it = persist_claim(a.ctr + (2 * s) * kXcds + xcd, &s_item);
+      if (it >= (hi - lo) * cch) break;
+      const int j = lo + it / cch, ch = it % cch;
+      const int tile = (skipped > 0 && j >= a.tile_lo) ? j + skipped : j;
+      const int b0 = ch * a.col_pb, b1 = min(a.nprobe, b0 + a.col_pb);
+      persist_col_item<N, ZK>(a, tile, b0, b1, smem);
+    }
+    if (!persist_barrier(count, err, k++, a.spin_limit, &s_flag)) return;
+    if (s + 1 == a.S - 1) break;
+    const float2* t_sl = a.t + (long long)(s + 1) * N * N;
+    persist_band(row_units, xcd, &lo, &hi);
+    for (;;) {
+      const int it = persist_claim(a.ctr + (2 * s + 1) * kXcds + xcd, &s_item);
+      if (it >= (hi - lo) * rch) break;
+      const int rb = lo + it / rch, ch = it % rch;
+      const int b0 = ch * a.row_pb, b1 = min(a.nprobe, b0 + a.row_pb);
+      persist_row_item<N, ZK>(a, t_sl, rb, b0, b1, true, true, smem, tab);
+    }
+    if (!persist_barrier(count, err, k++, a.spin_limit, &s_flag)) return;
+  }
+}
+
+// zk_x, zk_y: the plan's pruned slots (the kernel's items are all pruned)
+bool persist_supported(int nx, int ny, int zk_x, int zk_y) {
+  return nx == ny && nx == 1024 && PersistCfg<1024>::ok && zk_x >= PrunedSlots<1024>::value &&
+         zk_y >= PrunedSlots<1024>::value;
+}
+
+int persist_counters(int S) { return 2 * S * kXcds + 1; }
+
+hipError_t persist_grid(int n, int* grid) {
+  if (n != 1024) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void*)persist_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)PersistCfg<1024>::LDS);
+  if (e != hipSuccess) return e;
+  int dev = 0, cus = 0, occ = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, persist_kernel<1024>, kPersistThreads,
+                                                        PersistCfg<1024>::LDS)) != hipSuccess)
+    return e;
+  *grid = (occ * cus) / kXcds * kXcds;
+  return *grid > 0 ? hipSuccess : hipErrorInvalidConfiguration;
+}
+
+hipError_t launch_persist(const PersistArgs& args, int grid, hipStream_t s) {
+  if (args.n != 1024 || grid <= 0 || grid % kXcds != 0 || args.row_pb <= 0 || args.col_pb <= 0 || args.S <= 0)
+    return hipErrorInvalidValue;
+  PersistArgs a = args;
+  void* params[] = {&a};
+  return hipLaunchCooperativeKernel((const void*)persist_kernel<1024>, dim3(grid), dim3(kPersistThreads), params,
+                                    (unsigned)PersistCfg<1024>::LDS, s);
+}
+
+// --------------------------------------------------------------------------
 // Mode tables: every (size, mode) the host layer launches is instantiated here.
 // --------------------------------------------------------------------------
 #define EM_ROW_MODES(X)                                                         \

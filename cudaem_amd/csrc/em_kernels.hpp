@@ -136,6 +136,35 @@ struct ColArgs {
 
 enum { COL_VARIANT_BLOCK = 0, COL_VARIANT_TRANSPOSED = 1 };
 
+// --- persistent slice loop -----------------------------------------------------
+// One launch over a probe group: the propagator column passes of slices
+// 0..S-2 and the row passes of slices 1..S-2 between them (all pruned; the
+// caller runs slice 0's row pass before and slice S-1's after).  S >= 2.
+// ctr: persist_counters(S) ints, zero before the
+// launch (per-XCD work queues of every phase, the barrier count); *err is set
+// to 1 when a barrier wait passes spin_limit (every workgroup then leaves).
+struct PersistArgs {
+  float2* wave;             // [nprobe] waves, pstride apart, (y, kx) layout
+  long long pstride;
+  int n, nprobe, S;
+  const float2* t;          // [S][n][n] transmissions
+  const float2* P;          // [n][n]
+  const float2* twx;
+  const float2* twy;
+  int skip_lo, skip_hi;     // band-limit skip (row passes)
+  int tile_lo, tile_hi;     // 8-column tiles wholly inside it (column passes)
+  int zk_x, zk_y;           // line slots the pruned items may drop
+  int row_pb, col_pb;       // probes per work item
+  int* ctr;
+  int* err;
+  long long spin_limit;     // wall-clock ticks (100 MHz) of one barrier wait
+};
+bool persist_supported(int nx, int ny, int zk_x, int zk_y);
+int persist_counters(int S);
+// workgroups of a launch (all resident: occupancy x CUs, a multiple of 8)
+hipError_t persist_grid(int n, int* grid);
+hipError_t launch_persist(const PersistArgs& a, int grid, hipStream_t s);
+
 hipError_t launch_row(unsigned mode, const RowArgs& a, hipStream_t s);
 hipError_t launch_col(unsigned mode, const ColArgs& a, hipStream_t s);
 // Row-pass workgroups of this mode and row length that one CU holds at once

@@ -112,9 +112,14 @@ em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* 
  * 2 = one wavefront per column after the first stage); row_variant: row
  * pass of the STEM slice loop (0 = precomputed transmission held in
  * registers, 1 = read per probe, four waves per SIMD, 2 = potential slice
- * read and its sincos held in registers: half the table traffic). */
+ * read and its sincos held in registers: half the table traffic);
+ * persistent: 1 = run the interior of a probe group's slice loop on
+ * precomputed transmissions (the propagator passes of slices 0..S-2 and the
+ * row passes between them) as one cooperative launch with grid barriers
+ * (1024^2 grids with the band-limit skip on; the call then waits for its
+ * work to finish, and reports a barrier wait that timed out as EM_ERR_HIP). */
 typedef struct em_tuning {
-  int row_blocks, col_blocks, col_tile, probe_group, col_variant, row_variant;
+  int row_blocks, col_blocks, col_tile, probe_group, col_variant, row_variant, persistent;
 } em_tuning;
 em_status em_plan_get_tuning(em_plan plan, em_tuning* tuning);
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* tuning);

@@ -837,6 +837,41 @@ def test_band_skip_is_below_fp32_resolution(em):
     assert rel_l2(res[1e-9][2], res[0.0][2]) < 1e-6
 
 
+def test_persistent_slice_loop_matches_passes(em):
+    """The persistent slice loop (one cooperative launch per probe group for
+    the interior passes) gives the separate passes' results: the forward loop
+    on precomputed transmissions and batched STEM at 1024^2, over several
+    probe groups; STEM sums against the oracle."""
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.srtio3(5, 5, 5)
+    n, dz, e0 = 1024, 1.9525, 300.0
+    rng = np.random.default_rng(31)
+    S = 6
+    t = torch.from_numpy(np.exp(1j * rng.uniform(0, 0.4, (S, n, n))).astype(np.complex64)).cuda()
+    lens = probe_lens(25.0)
+    xs, ys = rng.uniform(0, at["lx"], 12), rng.uniform(0, at["ly"], 12)
+    res = {}
+    with em.Plan(n, n, at["lx"], at["ly"], dz, e0, max_batch=12) as p:
+        p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"], border=em.BORDER_MIN)
+        psi0 = p.build_probes(lens, xs[:5], ys[:5])
+        for pers in (0, 1):
+            p.set_tuning(persistent=pers, probe_group=3, row_blocks=256, col_blocks=512)
+            psi = psi0.clone()
+            p.multislice_forward(psi, t=t)
+            res[pers] = (_np(psi), _np(p.stem(lens, xs, ys, [0.0, 40.0], [40.0, 200.0])))
+        torch.cuda.synchronize()
+    assert rel_l2(res[1][0], res[0][0]) < 1e-6
+    assert rel_l2(res[1][1], res[0][1]) < 1e-6
+    o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
+    g2 = [(np.sin(np.array(a) * 1e-3) / o.lam) ** 2 for a in ([0.0, 40.0], [40.0, 200.0])]
+    ref, _ = O.stem(n, n, at["lx"], at["ly"], o.lam, lens, o.zmid0(), o.propagator(), o.potentials(), xs[:4],
+                    ys[:4], g2[0], g2[1], nthreads=4)
+    e = rel_l2(res[1][1][:, :4], ref)
+    print("persistent STEM relL2", e)
+    assert e < TOL_INT
+
+
 def test_stem_graph_cache_follows_changes(em):
     """em_stem keeps its captured batch graphs across calls: a repeated call
     is bitwise identical; a new specimen is read by the kept graph; launch
