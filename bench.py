@@ -40,8 +40,8 @@ HBM_PEAK = 8.0e12
 # the launch shapes they were measured at.  The bench reports it only when its
 # own tuning is the same: other shapes move other bytes.
 PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
-              "probe_group": 37}
-PMC_BYTES_PER_PROBE = {"column pass": (265.74 + 229.18) / 37 * 2**20, "row pass": (230.21 + 222.92) / 37 * 2**20}
+              "persistent": 0, "probe_group": 37}
+PMC_BYTES_PER_PROBE = {"column pass": (254.85 + 228.52) / 37 * 2**20, "row pass": (230.20 + 222.00) / 37 * 2**20}
 
 
 def parse():
@@ -62,6 +62,9 @@ def parse():
                     help="column-pass workgroup target instead of the default (with --no-autotune)")
     ap.add_argument("--plain-probes", type=int, default=32,
                     help="probes of the one-probe-per-launch side of batched_vs_plain (each grid)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="run the interior of each probe group's slice loop as one persistent launch "
+                         "(with --only-stem)")
     ap.add_argument("--no-band-skip", action="store_true",
                     help="compute every propagator column (no band-limit skipping)")
     ap.add_argument("--only-stem", action="store_true",
@@ -316,6 +319,9 @@ def ragged_mix(em, torch, n_req=8, seed=7, reps=3):
 def main():
     global N_GRID
     args = parse()
+    if args.persistent and not (args.only_stem and args.no_autotune):
+        print(json.dumps({"error": "--persistent needs --only-stem --no-autotune"}))
+        sys.exit(2)
     if args.grid != N_GRID and not args.only_stem:
         print(json.dumps({"error": "--grid other than 1024 needs --only-stem (the roofline's traffic "
                                    "constants are measured at 1024^2)"}))
@@ -359,6 +365,8 @@ def main():
             plan.set_tuning(row_blocks=args.row_blocks)
         if args.col_blocks:
             plan.set_tuning(col_blocks=args.col_blocks)
+        if args.persistent:
+            plan.set_tuning(persistent=1)
     else:
         tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=8)
         if args.group:
@@ -426,7 +434,9 @@ def main():
     skip_col = tile * max(0, (hi + 1) // tile - -(-lo // tile)) if hi >= lo else 0
     row_bytes = 16 * N_GRID * (N_GRID - skip_row) * probes_per_launch + 8 * N
     col_bytes = 16 * N_GRID * (N_GRID - skip_col) * probes_per_launch + 8 * N_GRID * (N_GRID - skip_col)
-    shapes = {k: tun[k] for k in ("row_blocks", "col_blocks", "col_tile", "col_variant", "row_variant")}
+    shapes = {k: tun[k] for k in ("row_blocks", "col_blocks", "col_tile", "col_variant", "row_variant",
+                                  "persistent")}
+    shapes["col_variant"] = shapes["col_variant"] or 1  # 0 = automatic = the block exchange (1)
     shapes["probe_group"] = G
     profiled = all(PMC_SHAPES[k] == v for k, v in shapes.items())
     pmc = {k: v * probes_per_launch for k, v in PMC_BYTES_PER_PROBE.items()} if profiled else None
