@@ -40,7 +40,7 @@ HBM_PEAK = 8.0e12
 # the launch shapes they were measured at.  The bench reports it only when its
 # own tuning is the same: other shapes move other bytes.
 PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
-              "persistent": 0, "probe_group": 37}
+              "persistent": 0, "row_radix": 0, "col_radix": 0, "probe_group": 37}
 PMC_BYTES_PER_PROBE = {"column pass": (254.85 + 228.52) / 37 * 2**20, "row pass": (230.20 + 222.00) / 37 * 2**20}
 
 
@@ -435,7 +435,7 @@ def main():
     row_bytes = 16 * N_GRID * (N_GRID - skip_row) * probes_per_launch + 8 * N
     col_bytes = 16 * N_GRID * (N_GRID - skip_col) * probes_per_launch + 8 * N_GRID * (N_GRID - skip_col)
     shapes = {k: tun[k] for k in ("row_blocks", "col_blocks", "col_tile", "col_variant", "row_variant",
-                                  "persistent")}
+                                  "persistent", "row_radix", "col_radix")}
     shapes["col_variant"] = shapes["col_variant"] or 1  # 0 = automatic = the block exchange (1)
     shapes["probe_group"] = G
     profiled = all(PMC_SHAPES[k] == v for k, v in shapes.items())

@@ -67,7 +67,7 @@ class em_coherence(ctypes.Structure):
 class em_tuning(ctypes.Structure):
     _fields_ = [("row_blocks", ctypes.c_int), ("col_blocks", ctypes.c_int), ("col_tile", ctypes.c_int),
                 ("probe_group", ctypes.c_int), ("col_variant", ctypes.c_int), ("row_variant", ctypes.c_int),
-                ("persistent", ctypes.c_int)]
+                ("persistent", ctypes.c_int), ("row_radix", ctypes.c_int), ("col_radix", ctypes.c_int)]
 
 
 class em_atoms(ctypes.Structure):

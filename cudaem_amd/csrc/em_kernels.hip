@@ -79,11 +79,13 @@ __device__ __forceinline__ UniformChunk read_chunk(const PassChunk* c) {
 // --------------------------------------------------------------------------
 // FFT configurations per line length: threads per line and radix sequence.
 // --------------------------------------------------------------------------
-template <int N>
+// V: the radix sequence (0 = the default of the size; others are autotuner
+// alternatives of the pruned slice-loop passes, FftVariants<N>::count).
+template <int N, int V = 0>
 struct FftCfg;
-#define EM_FFT_CFG(N_, T_, ...)                                                     \
+#define EM_FFT_CFG(N_, V_, T_, ...)                                                 \
   template <>                                                                       \
-  struct FftCfg<N_> {                                                               \
+  struct FftCfg<N_, V_> {                                                           \
     static constexpr int T = T_;                                                    \
     using Tw = TwChain<N_, T_, 1, __VA_ARGS__>;                                     \
     using TwTab = TwChainTab<N_, T_, 1, __VA_ARGS__>;                               \
@@ -105,14 +107,26 @@ struct FftCfg;
       FftLine<N_, T_, DIR, SYNC>::template rest<__VA_ARGS__>(v, lds, tau, tws, tw); \
     }                                                                               \
   };
-EM_FFT_CFG(64, 8, 8, 8)
-EM_FFT_CFG(128, 16, 8, 8, 2)
-EM_FFT_CFG(256, 16, 16, 16)
-EM_FFT_CFG(512, 64, 8, 8, 8)
-EM_FFT_CFG(768, 64, 12, 4, 4, 4)
-EM_FFT_CFG(1024, 64, 16, 16, 4)
-EM_FFT_CFG(2048, 128, 16, 16, 8)
+EM_FFT_CFG(64, 0, 8, 8, 8)
+EM_FFT_CFG(128, 0, 16, 8, 8, 2)
+EM_FFT_CFG(256, 0, 16, 16, 16)
+EM_FFT_CFG(512, 0, 64, 8, 8, 8)
+EM_FFT_CFG(768, 0, 64, 12, 4, 4, 4)
+EM_FFT_CFG(1024, 0, 64, 16, 16, 4)
+EM_FFT_CFG(1024, 1, 64, 16, 4, 16)
+EM_FFT_CFG(1024, 2, 64, 4, 16, 16)
+EM_FFT_CFG(2048, 0, 128, 16, 16, 8)
+EM_FFT_CFG(2048, 1, 128, 16, 8, 16)
+EM_FFT_CFG(2048, 2, 128, 8, 16, 16)
 #undef EM_FFT_CFG
+
+// Radix sequences of a size: three stages whose radices divide the values
+// per lane (16) at 1024 and 2048; one elsewhere (512: only 8.8.8 has three
+// stages; 768 needs its factor 3 in a radix dividing 12).
+template <int N>
+struct FftVariants {
+  static constexpr int count = (N == 1024 || N == 2048) ? 3 : 1;
+};
 
 constexpr int kRowThreads = 256;
 
@@ -166,7 +180,7 @@ template <int N>
 struct AltCols {
   static constexpr int value = (FftCfg<N>::T >= 128) ? 4 : 16;
 };
-template <int N, int COLS_ = DefaultCols<N>::value>
+template <int N, int COLS_ = DefaultCols<N>::value, int V = 0>
 struct ColCfg {
   static constexpr int T = FftCfg<N>::T;
   static constexpr int COLS = COLS_;
@@ -182,8 +196,8 @@ struct ColCfg {
   // Twiddles from an LDS table after the column buffers where the register
   // form would read the global table at every use (768); elsewhere they are
   // formed in registers.
-  static constexpr bool TWTAB = !FftCfg<N>::Tw::in_registers;
-  static constexpr int TAB = TWTAB ? FftCfg<N>::TwTab::entries : 0;
+  static constexpr bool TWTAB = !FftCfg<N, V>::Tw::in_registers;
+  static constexpr int TAB = TWTAB ? FftCfg<N, V>::TwTab::entries : 0;
   static constexpr size_t LDS = ((size_t)COLS * STRIDE + TAB) * sizeof(float2);
 };
 
@@ -198,10 +212,10 @@ struct RowOcc {
 };
 
 // ZK > 0: the pruned form of a slice-loop pass (skip_in and skip_out set).
-template <int NX, unsigned MODE, int ZK = 0>
+template <int NX, unsigned MODE, int ZK = 0, int V = 0>
 __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(RowOcc<MODE>::waves)))
 void row_pass_kernel(RowArgs a) {
-  constexpr int T = FftCfg<NX>::T;
+  constexpr int T = FftCfg<NX, V>::T;
   constexpr int E = NX / T;
   constexpr int LINES = kRowThreads / T;
   extern __shared__ float2 smem[];
@@ -226,11 +240,11 @@ void row_pass_kernel(RowArgs a) {
   constexpr int SYNC = (T <= 64) ? SYNC_WAVE : SYNC_BLOCK;  // a line inside one wavefront
   // Twiddles: a per-workgroup LDS table after the line buffers where every
   // stage allows it (one read per twiddle instead of forming it in registers)
-  constexpr bool TWTAB = FftCfg<NX>::TwTab::ok;
-  std::conditional_t<TWTAB, typename FftCfg<NX>::TwTab, typename FftCfg<NX>::Tw> tws;
+  constexpr bool TWTAB = FftCfg<NX, V>::TwTab::ok;
+  std::conditional_t<TWTAB, typename FftCfg<NX, V>::TwTab, typename FftCfg<NX, V>::Tw> tws;
   if constexpr (TWTAB) {
     float2* tab = smem + LINES * LineLds<NX>::size;
-    FftCfg<NX>::TwTab::fill(tab, a.tw, threadIdx.x, kRowThreads);
+    FftCfg<NX, V>::TwTab::fill(tab, a.tw, threadIdx.x, kRowThreads);
     __syncthreads();
     tws.init(tau, tab);
   } else {
@@ -292,7 +306,7 @@ void row_pass_kernel(RowArgs a) {
     } else {
       load_row(base, v);
     }
-    if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX, V>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) a.ckpt[base + tau + T * e] = v[e];
@@ -313,7 +327,7 @@ void row_pass_kernel(RowArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = cmul(v[e], t[e]);
     }
-    if constexpr ((MODE & ROW_POST_FFT) != 0) FftCfg<NX>::template run<-1, SYNC>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & ROW_POST_FFT) != 0) FftCfg<NX, V>::template run<-1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_OUT_ABS2) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -374,10 +388,10 @@ struct ColLateP {
 // 8-column tile then share a CU, one loading while the other computes.
 // ZK > 0: the pruned form of a propagator pass (ky slots wholly inside the
 // band-limit skip: their propagator is below the skip threshold).
-template <int NY, int COLS_, unsigned MODE, int ZK = 0>
-__global__ __launch_bounds__((ColCfg<NY, COLS_>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+template <int NY, int COLS_, unsigned MODE, int ZK = 0, int V = 0>
+__global__ __launch_bounds__((ColCfg<NY, COLS_, V>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
 void col_pass_kernel(ColArgs a) {
-  using C = ColCfg<NY, COLS_>;
+  using C = ColCfg<NY, COLS_, V>;
   constexpr int T = C::T;
   constexpr int E = NY / T;
   constexpr int COLS = C::COLS;
@@ -414,10 +428,10 @@ void col_pass_kernel(ColArgs a) {
     b1 = ch.n;
   }
   float2* lds = smem + c * C::STRIDE;
-  std::conditional_t<C::TWTAB, typename FftCfg<NY>::TwTab, typename FftCfg<NY>::Tw> tws;
+  std::conditional_t<C::TWTAB, typename FftCfg<NY, V>::TwTab, typename FftCfg<NY, V>::Tw> tws;
   if constexpr (C::TWTAB) {
     float2* tab = smem + COLS * C::STRIDE;
-    FftCfg<NY>::TwTab::fill(tab, a.tw, threadIdx.x, C::THREADS);
+    FftCfg<NY, V>::TwTab::fill(tab, a.tw, threadIdx.x, C::THREADS);
     __syncthreads();
     tws.init(tau, tab);
   } else {
@@ -497,8 +511,8 @@ void col_pass_kernel(ColArgs a) {
         for (int e = 0; e < E; ++e) v[e] = cmul(v[e], p[e]);
       }
     }
-    if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
-    if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_PRE_FFT) != 0) FftCfg<NY, V>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_PRE_IFFT) != 0) FftCfg<NY, V>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr (LATE_P) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -539,8 +553,8 @@ void col_pass_kernel(ColArgs a) {
       }
       __syncthreads();
     }
-    if constexpr ((MODE & COL_POST_FFT) != 0) FftCfg<NY>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
-    if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_POST_FFT) != 0) FftCfg<NY, V>::template run<-1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
+    if constexpr ((MODE & COL_POST_IFFT) != 0) FftCfg<NY, V>::template run<+1, SYNC_BLOCK>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & COL_STORE) != 0) {
 #pragma unroll
       for (int e = 0; e < E; ++e) out[base + (long long)(tau + T * e) * nx] = v[e];
@@ -937,6 +951,18 @@ hipError_t launch_persist(const PersistArgs& args, int grid, hipStream_t s) {
 
 #define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
 
+int fft_variants(int n) {
+  switch (n) {
+#define EM_CASE(N) \
+  case N:          \
+    return FftVariants<N>::count;
+    EM_SIZES(EM_CASE)
+#undef EM_CASE
+    default:
+      return 0;
+  }
+}
+
 bool fft_size_supported(int n) {
   switch (n) {
 #define EM_CASE(N) case N:
@@ -949,10 +975,10 @@ bool fft_size_supported(int n) {
 }
 
 // dynamic LDS of a row-pass workgroup: its line buffers and twiddle table
-template <int NX>
+template <int NX, int V = 0>
 constexpr size_t row_lds() {
   constexpr int LINES = kRowThreads / FftCfg<NX>::T;
-  constexpr int TAB = FftCfg<NX>::TwTab::ok ? FftCfg<NX>::TwTab::entries : 0;
+  constexpr int TAB = FftCfg<NX, V>::TwTab::ok ? FftCfg<NX, V>::TwTab::entries : 0;
   return ((size_t)LINES * LineLds<NX>::size + TAB) * sizeof(float2);
 }
 
@@ -963,6 +989,13 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
   constexpr size_t lds = row_lds<NX>();
   constexpr int ZK = PrunedSlots<NX>::value;
   if (ZK > 0 && a.zk >= ZK && a.skip_in && a.skip_out) {
+    if constexpr (FftVariants<NX>::count > 1) {
+      constexpr unsigned HOT = ROW_PRE_IFFT | ROW_MUL_T | ROW_POST_FFT;
+      if (mode == HOT && a.radix == 1)
+        return launch_pass(row_pass_kernel<NX, HOT, ZK, 1>, grid, dim3(kRowThreads), row_lds<NX, 1>(), s, a);
+      if (mode == HOT && a.radix == 2)
+        return launch_pass(row_pass_kernel<NX, HOT, ZK, 2>, grid, dim3(kRowThreads), row_lds<NX, 2>(), s, a);
+    }
     switch (mode) {
 #define EM_ROW_PRUNED_CASE(M) \
   case (M):                   \
@@ -997,13 +1030,13 @@ static int skipped_tiles(unsigned mode, ColArgs& a) {
   return hi - lo + 1;
 }
 
-template <int NY, int COLS, unsigned M, int ZK = 0>
+template <int NY, int COLS, unsigned M, int ZK = 0, int V = 0>
 static hipError_t col_launch_one(const ColArgs& args, hipStream_t s) {
-  using C = ColCfg<NY, COLS>;
+  using C = ColCfg<NY, COLS, V>;
   ColArgs a = args;
   const int skipped = skipped_tiles<COLS>(M, a);
   const dim3 grid(a.nx / C::COLS - skipped, (a.nprobe + a.pb - 1) / a.pb);
-  return launch_pass(col_pass_kernel<NY, COLS, M, ZK>, grid, dim3(C::THREADS), C::LDS, s, a);
+  return launch_pass(col_pass_kernel<NY, COLS, M, ZK, V>, grid, dim3(C::THREADS), C::LDS, s, a);
 }
 
 template <int NY>
@@ -1041,6 +1074,17 @@ static hipError_t col_attrs(void) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ColCfg<NY>::LDS);
     EM_COL_PRUNED_CHUNK_MODES(EM_COL_ATTR_PRC)
 #undef EM_COL_ATTR_PRC
+    if constexpr (FftVariants<NY>::count > 1) {
+      constexpr unsigned HOT = COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE;
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, HOT, ZK, 1>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ColCfg<NY, DefaultCols<NY>::value, 1>::LDS);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, HOT, ZK, 2>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ColCfg<NY, DefaultCols<NY>::value, 2>::LDS);
+    }
   }
   if constexpr (FftCfg<NY>::T == 64) {
 #define EM_COL_ATTR_T(M)                                                                 \
@@ -1111,6 +1155,11 @@ static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
   constexpr int ZK = PrunedSlots<NY>::value;
   if (ZK > 0 && a.zk >= ZK) {
     const bool alt = a.cols == AltCols<NY>::value;
+    if constexpr (FftVariants<NY>::count > 1) {
+      constexpr unsigned HOT = COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE;
+      if (mode == HOT && !alt && a.radix == 1) return col_launch_one<NY, DefaultCols<NY>::value, HOT, ZK, 1>(a, s);
+      if (mode == HOT && !alt && a.radix == 2) return col_launch_one<NY, DefaultCols<NY>::value, HOT, ZK, 2>(a, s);
+    }
     switch (mode) {
 #define EM_COL_PR_CASE(M)                                                                   \
   case (M):                                                                                 \

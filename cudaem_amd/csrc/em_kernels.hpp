@@ -83,6 +83,7 @@ struct RowArgs {
   // slots per line end wholly inside [skip_lo, skip_hi] (pruned_slots): with
   // skip_in and skip_out set, the slice-loop modes then run pruned
   int zk;
+  int radix;                // radix sequence of the pruned STEM row pass (fft_variants)
   const PassChunk* chunks;  // ROW_CHUNKS: one entry per workgroup row
 };
 
@@ -131,6 +132,7 @@ struct ColArgs {
   // propagator modes: ky slots wholly inside the band-limit skip range of
   // the column (pruned_slots of ny and that range); > 0 runs them pruned
   int zk;
+  int radix;                // radix sequence of the pruned propagator pass (fft_variants)
   const PassChunk* chunks;  // COL_CHUNKS: one entry per workgroup row
 };
 
@@ -180,6 +182,8 @@ bool pass_events_pending();
 // (call once per device before launching them).
 hipError_t prepare_col_kernels(int ny);
 bool fft_size_supported(int n);
+// radix sequences a line length is built with (tuning row_radix / col_radix)
+int fft_variants(int n);
 int row_lines_per_block(int nx);
 int col_tile(int ny);      // default columns per column-pass workgroup
 int col_alt_tile(int ny);  // the alternative tile of the per-slice column modes

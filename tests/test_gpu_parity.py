@@ -837,6 +837,38 @@ def test_band_skip_is_below_fp32_resolution(em):
     assert rel_l2(res[1e-9][2], res[0.0][2]) < 1e-6
 
 
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_radix_variants_agree(em, n):
+    """The autotuner's radix sequences of the row and column lines (1024:
+    16.16.4, 16.4.16, 4.16.16; 2048: 16.16.8, 16.8.16, 8.16.16) give the
+    default sequence's exit waves to FP32 rounding, and the default matches
+    the oracle."""
+    import torch
+    rng = np.random.default_rng(41)
+    S, B, L = 4, 3, 20.0 * n / 1024
+    t = np.exp(1j * rng.uniform(0, 0.5, (S, n, n)))
+    tt = torch.from_numpy(t.astype(np.complex64)).cuda()
+    with em.Plan(n, n, L, L, 2.0, 300.0, max_batch=B) as p:
+        assert p.set_band_skip(1e-9)[0] > 0
+        psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, L, B), rng.uniform(0, L, B))
+        outs = {}
+        for rr in range(3):
+            for cr in range(3):
+                p.set_tuning(row_radix=rr, col_radix=cr)
+                psi = psi0.clone()
+                p.multislice_forward(psi, t=tt)
+                outs[(rr, cr)] = _np(psi)
+        P = _np(p.propagator()).astype(np.complex128)
+        p0 = _np(psi0[0]).astype(np.complex128)
+        torch.cuda.synchronize()
+    ref = outs[(0, 0)]
+    for key, v in outs.items():
+        assert rel_l2(v, ref) < 2e-6, key
+    e = rel_l2(ref[0], O.multislice(P, p0, t=t))
+    print("radix variants: default vs oracle relL2", e)
+    assert e < TOL_WAVE
+
+
 def test_persistent_slice_loop_matches_passes(em):
     """The persistent slice loop (one cooperative launch per probe group for
     the interior passes) gives the separate passes' results: the forward loop
