@@ -42,6 +42,12 @@ HBM_PEAK = 8.0e12
 PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
               "persistent": 0, "row_radix": 0, "col_radix": 0, "probe_group": 37}
 PMC_BYTES_PER_PROBE = {"column pass": (254.85 + 228.52) / 37 * 2**20, "row pass": (230.20 + 222.00) / 37 * 2**20}
+T_START = time.perf_counter()
+
+
+def note(msg):
+    """One progress line per finished stage, on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -371,6 +377,7 @@ def main():
         tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=8)
         if args.group:
             plan.set_tuning(probe_group=args.group)
+        note(f"autotune done: {plan.tuning()}")
     lens = {"outer_aper_mrad": ALPHA}
     out = torch.zeros((1, npos), dtype=torch.float32, device="cuda")
 
@@ -398,6 +405,7 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ps = args.steps * npos * S / dt
+    note(f"{args.steps} timed STEM steps: {ps:.0f} probe-slices/s")
 
     if args.only_stem:
         print(json.dumps({"metric": "probe-slices/s (batched STEM steps only)", "value": ps, "steps": args.steps,
@@ -420,6 +428,7 @@ def main():
     row_ms, col_ms, n_row, n_col = plan.pass_times()
     plan.set_pass_timing(False)
     step_ms = row_ms + col_ms
+    note(f"event-timed batch: row {row_ms:.4f} ms, column {col_ms:.4f} ms")
     # Algorithmic bytes of one slice step over a group: per probe-slice one
     # read and one write of the wave (16 B per pixel) plus the shared potential
     # (4 B) and propagator (8 B) per pixel amortised over the group.
@@ -448,18 +457,23 @@ def main():
     rng = np.random.default_rng(SEED)
     sample = rng.choice(len(xs), args.plain_probes, replace=False)
     plain_ps = plain_path(em, torch, at1, xs[sample], ys[sample], args.plain_probes, N_GRID)
+    note(f"plain path at {N_GRID}^2: {plain_ps:.0f} probe-slices/s")
     xs32, ys32 = em.specimen.raster(32, 32, L, L)
     sample32 = rng.choice(len(xs32), args.plain_probes, replace=False)
     plain512 = plain_path(em, torch, at0, xs32[sample32], ys32[sample32], args.plain_probes, 512)
     batched512 = batched_steps(em, torch, at0, xs32, ys32, 512)
+    note(f"512^2: plain {plain512:.0f}, batched {batched512:.0f} probe-slices/s")
     t_fwd, t_fwd_adj = adjoint_ratio(em, torch)
+    note(f"forward+adjoint / forward: {t_fwd_adj / t_fwd:.3f}")
     rag_ps, rag_work = ragged_mix(em, torch)
+    note(f"ragged mix: {rag_ps:.0f} probe-slices/s")
 
     cpu = None
     if not args.no_cpu:
         threads, nproc = cpu_cores()
         n_cpu = args.cpu_probes or threads
         v, used, rel, units, fp32 = cpu_baseline(at1, xs, ys, sums, n_cpu, threads)
+        note(f"CPU baseline: {v:.1f} probe-slices/s on {used} threads")
         full_s = args.steps * npos * S / v
         cpu = {"value": v, "unit": "probe-slices/s", "cores": used, "kind": "port",
                "sample": f"{n_cpu} probes x {S} slices of phonon configuration 1 ({units} probe-slices, FP64 "
