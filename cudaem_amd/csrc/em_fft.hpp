@@ -133,6 +133,84 @@ struct Dft<4, DIR, B, S, E> {
 };
 
 // Composite R = R1*R2 by the four-step split n = R2*n1 + n2, k = k1 + R1*k2.
+// A compile-time twiddle w = exp(DIR 2 pi i M / N) in the factored form
+// w = g (1 + i tau) (g = cos, tau = tan of its angle), so that w x = g u with
+// u = x + i tau x: two FMAs (two adds when tau = +-1), and the real g can
+// ride on the FMAs of the butterfly that consumes w x.  Multiples of pi/2
+// keep g = 1 and make u a swap / negation.
+template <int N, int M, int DIR>
+struct TanTw {
+  static constexpr int m = ((M % N) + N) % N;
+  // 0: w = 1, 1: w = DIR i, 2: w = -1, 3: w = -DIR i, 4: general
+  static constexpr int kind = m == 0 ? 0 : (4 * m == N ? 1 : (2 * m == N ? 2 : (4 * m == 3 * N ? 3 : 4)));
+  static constexpr double ang = 2.0 * kPiD * (double)m / N;
+  static constexpr double c = cx_cos(ang);
+  static constexpr double s = DIR * cx_sin(ang);
+  static constexpr double g = kind == 4 ? c : 1.0;
+  // odd multiples of pi/4: tau = +-1 exactly
+  static constexpr bool unit = kind == 4 && (8 * m) % N == 0 && ((8 * m) / N) % 2 == 1;
+  static constexpr float tau = kind == 4 ? (float)(s / c) : 0.f;
+  __device__ __forceinline__ static float2 u(float2 x) {
+    if constexpr (kind == 0) {
+      return x;
+    } else if constexpr (kind == 1) {
+      return mul_i<DIR>(x);
+    } else if constexpr (kind == 2) {
+      return make_float2(-x.x, -x.y);
+    } else if constexpr (kind == 3) {
+      return mul_i<-DIR>(x);
+    } else if constexpr (unit) {
+      return tau > 0.f ? make_float2(x.x - x.y, x.y + x.x) : make_float2(x.x + x.y, x.y - x.x);
+    } else {
+      return make_float2(__builtin_fmaf(-tau, x.y, x.x), __builtin_fmaf(tau, x.x, x.y));
+    }
+  }
+};
+
+// a + k b and a - k b for a compile-time real k (adds when k = +-1)
+template <int K_IS>  // 1: k = 1, -1: k = -1, 0: general
+__device__ __forceinline__ void fma_pm(float2 a, float2 b, float k, float2& p, float2& q) {
+  if constexpr (K_IS == 1) {
+    p = cadd(a, b);
+    q = csub(a, b);
+  } else if constexpr (K_IS == -1) {
+    p = csub(a, b);
+    q = cadd(a, b);
+  } else {
+    p = make_float2(__builtin_fmaf(k, b.x, a.x), __builtin_fmaf(k, b.y, a.y));
+    q = make_float2(__builtin_fmaf(-k, b.x, a.x), __builtin_fmaf(-k, b.y, a.y));
+  }
+}
+constexpr int unit_sign(double k) { return k == 1.0 ? 1 : (k == -1.0 ? -1 : 0); }
+
+// Radix-4 butterfly over x_n = v[B + n S] (n = 0..3) after the constant
+// twiddles w_n = exp(DIR 2 pi i n K / N), outputs in natural order in place.
+// With w_n x_n = g_n u_n:  t0,1 = u0 +- g2 u2;  q+- = u1 +- (g3/g1) u3;
+// y0,2 = t0 +- g1 q+;  y1,3 = t1 +- g1 (DIR i q-).  Every twiddle other than
+// a multiple of pi/2 costs 2 instructions (u) instead of a complex multiply's
+// 4; the g's are FMA operands of adds the butterfly makes anyway.
+template <int N, int K, int DIR, int B, int S, int E>
+struct Radix4Tw {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) {
+    using W1 = TanTw<N, K, DIR>;
+    using W2 = TanTw<N, 2 * K, DIR>;
+    using W3 = TanTw<N, 3 * K, DIR>;
+    const float2 u0 = v[B], u1 = W1::u(v[B + S]), u2 = W2::u(v[B + 2 * S]), u3 = W3::u(v[B + 3 * S]);
+    float2 t0, t1, qp, qm;
+    fma_pm<unit_sign(W2::g)>(u0, u2, (float)W2::g, t0, t1);
+    constexpr double r = W3::g / W1::g;
+    fma_pm<unit_sign(r)>(u1, u3, (float)r, qp, qm);
+    const float2 qmi = mul_i<DIR>(qm);
+    float2 y0, y1, y2, y3;
+    fma_pm<unit_sign(W1::g)>(t0, qp, (float)W1::g, y0, y2);
+    fma_pm<unit_sign(W1::g)>(t1, qmi, (float)W1::g, y1, y3);
+    v[B] = y0;
+    v[B + S] = y1;
+    v[B + 2 * S] = y2;
+    v[B + 3 * S] = y3;
+  }
+};
+
 template <int R1, int R2, int DIR, int B, int S, int E>
 struct DftComposite {
   static constexpr int R = R1 * R2;
@@ -141,6 +219,14 @@ struct DftComposite {
     if constexpr (N2 < R2) {
       Dft<R1, DIR, B + N2 * S, R2 * S, E>::run(v);
       step1<N2 + 1>(v);
+    }
+  }
+  // R2 = 4: the twiddles of step 2 folded into step 3's radix-4 butterflies
+  template <int K1>
+  __device__ __forceinline__ static void step23(float2 (&v)[E]) {
+    if constexpr (K1 < R1) {
+      Radix4Tw<R, K1, DIR, B + K1 * R2 * S, S, E>::run(v);
+      step23<K1 + 1>(v);
     }
   }
   template <int N2, int K1>
@@ -164,8 +250,12 @@ struct DftComposite {
   }
   __device__ __forceinline__ static void run(float2 (&v)[E]) {
     step1<0>(v);
-    step2<0, 0>(v);
-    step3<0>(v);
+    if constexpr (R2 == 4) {
+      step23<0>(v);
+    } else {
+      step2<0, 0>(v);
+      step3<0>(v);
+    }
     // X[k1 + R1 k2] sits at slot R2 k1 + k2: reorder to natural order
     float2 tmp[R];
 #pragma unroll
