@@ -130,8 +130,9 @@ em_status em_plan_get_tuning(em_plan plan, em_tuning* tuning);
 em_status em_plan_set_tuning(em_plan plan, const em_tuning* tuning);
 /* Autotuner: times candidate tunings (coordinate descent, two sweeps) by
  * running nslices slices of nprobe synthetic probes on the plan's grid with
- * HIP events and keeps the fastest; best_ms_per_probe_slice receives its
- * time.  Allocates and frees its own scratch. */
+ * HIP events, then re-times the fastest against the plan's starting tuning
+ * and keeps it only if it is at least 2% faster; best_ms_per_probe_slice
+ * receives the kept tuning's time.  Allocates and frees its own scratch. */
 em_status em_plan_autotune(em_plan plan, int nprobe, int nslices, double* best_ms_per_probe_slice);
 /* Band-limit skipping (no reference counterpart).  With the band-width limit
  * on, the propagator of every column kx with bwl factor <= threshold at ky = 0
