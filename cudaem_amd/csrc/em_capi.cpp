@@ -184,6 +184,7 @@ struct em_plan_s {
   float bin = 0;
   // device
   DevBuf tw_x, tw_y, P, atoms, off, tables, rmax2;
+  DevBuf twt_x, twt_y;  // per-lane twiddle tables of radix variant 3 (1024 only)
   DevBuf wave, tmp, dpart, A, ex, ey, xe, ye, gpart, Vstore, Tstore, detout, seg, segwave;
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
   DevBuf pk_ctr, pk_err;  // persistent slice loop: work queues and barrier count, error flag
@@ -251,6 +252,70 @@ em_status upload_twiddles(em_plan p, DevBuf& buf, int n) {
   return EM_OK;
 }
 
+// Per-lane twiddle tables of radix variant 3 at 1024 (the kernels' TanRow1024
+// and TanCol1024 describe the layouts), built in FP64 and rounded once.  A
+// stage twiddle exp(DIR i theta) is stored as T = tan(theta) and g = cos(theta),
+// plus the ratios of g's that ride on the radix-4 FMAs.
+em_status upload_tangent_tables(em_plan p, DevBuf& buf, int n, bool rows) {
+  if (n != 1024) return EM_OK;
+  std::vector<float> h;
+  // radix 4 of stage size rns: T1, T2, T3, g1, g2, g3/g1, 0, 0 for class k
+  auto radix4 = [&h](int k, int step, int rns) {
+    double T[4], c[4];
+    for (int r = 1; r <= 3; ++r) {
+      const double th = 2.0 * kPi * (double)r * step * k / rns;
+      T[r] = std::tan(th);
+      c[r] = std::cos(th);
+    }
+    for (double x : {T[1], T[2], T[3], c[1], c[2], c[3] / c[1]}) h.push_back((float)x);
+  };
+  if (rows) {
+    // stage 2 (radix 16 over 256): 16 classes of 40 floats
+    auto G16 = [](int m) { return (m % 16) % 4 == 0 ? 1.0 : std::cos(2.0 * kPi * (m % 16) / 16.0); };
+    for (int k = 0; k < 16; ++k) {
+      double c[16], T[16];
+      for (int r = 0; r < 16; ++r) {
+        const double th = 2.0 * kPi * r * k / 256.0;
+        c[r] = std::cos(th);
+        T[r] = std::tan(th);
+      }
+      for (int r = 1; r < 16; ++r) h.push_back((float)T[r]);
+      h.push_back(0.f);
+      for (int n2 = 0; n2 < 4; ++n2) h.push_back((float)(c[n2 + 8] / c[n2]));
+      for (int n2 = 0; n2 < 4; ++n2) h.push_back((float)(c[n2 + 12] / c[n2 + 4]));
+      for (int n2 = 0; n2 < 4; ++n2) h.push_back((float)(c[n2 + 4] / c[n2]));
+      for (int k1 = 0; k1 < 4; ++k1) h.push_back((float)(c[2] * G16(2 * k1)));
+      for (int k1 = 0; k1 < 4; ++k1) h.push_back((float)((c[3] * G16(3 * k1)) / (c[1] * G16(k1))));
+      for (int k1 = 0; k1 < 4; ++k1) h.push_back((float)(c[1] * G16(k1)));
+    }
+    // stage 3 (radix 4 over 1024): 256 classes
+    for (int k = 0; k < 256; ++k) {
+      radix4(k, 1, 1024);
+      h.push_back(0.f);
+      h.push_back(0.f);
+    }
+  } else {
+    // stage 2 (radix 4 over 64): 16 classes
+    for (int k = 0; k < 16; ++k) {
+      radix4(k, 1, 64);
+      h.push_back(0.f);
+      h.push_back(0.f);
+    }
+    // stage 3 (radix 16 over 1024): 64 classes of lo_1..lo_3 and W^(4k)'s radix 4
+    for (int k = 0; k < 64; ++k) {
+      for (int m = 1; m <= 3; ++m) {
+        const double a = -2.0 * kPi * m * k / 1024.0;
+        h.push_back((float)std::cos(a));
+        h.push_back((float)std::sin(a));
+      }
+      radix4(k, 4, 1024);
+    }
+  }
+  EM_HIP(p, buf.ensure(sizeof(float) * h.size()));
+  EM_HIP(p, hipMemcpy(buf.p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  return EM_OK;
+}
+
 // Probes per workgroup: enough workgroups to fill 256 CUs several times over,
 // and as many probes as possible sharing one register-held t or P.
 int row_pb(const em_plan p, int B) {
@@ -308,6 +373,7 @@ RowArgs row_args(em_plan p, const float2* in, float2* out, int B) {
   a.nprobe = B;
   a.pb = row_pb(p, B);
   a.tw = p->tw_x.as<float2>();
+  a.twt = p->twt_x.as<float2>();
   a.wgt = 1.f;
   a.skip_lo = p->skip_lo;
   a.skip_hi = p->skip_hi;
@@ -338,6 +404,7 @@ ColArgs col_args(em_plan p, const float2* in, float2* out, int B) {
   a.pb = col_pb(p, B);
   a.P = p->P.as<float2>();
   a.tw = p->tw_y.as<float2>();
+  a.twt = p->twt_y.as<float2>();
   a.grid = p->gdev;
   a.cols = p->tune.col_tile;
   const bool tr = col_transposed_available(p->grid.ny) && (a.cols == 0 || a.cols == 16 || a.cols == 8);
@@ -600,6 +667,8 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
     EM_HIP(p.get(), hipEventCreateWithFlags(e, hipEventDisableTiming));
   EM_TRY(upload_twiddles(p.get(), p->tw_x, grid->nx));
   EM_TRY(upload_twiddles(p.get(), p->tw_y, grid->ny));
+  EM_TRY(upload_tangent_tables(p.get(), p->twt_x, grid->nx, true));
+  EM_TRY(upload_tangent_tables(p.get(), p->twt_y, grid->ny, false));
   // Propagator, built once per plan (wave_function.cuh:42-44)
   em_plan raw = p.get();
   EM_HIP(raw, p->P.ensure(sizeof(float2) * p->N()));
@@ -2026,7 +2095,7 @@ extern "C" em_status em_plan_autotune(em_plan p, int nprobe, int nslices, double
   const int persistents[] = {0, 1};
   const int npers = persist_supported(p->grid.nx, p->grid.ny, p->zk_x, p->zk_y) ? 2 : 1;
   // radix sequences of the row and column lines
-  const int radices[] = {0, 1, 2};
+  const int radices[] = {0, 1, 2, 3};
   const int nrx = fft_variants(p->grid.nx), nry = fft_variants(p->grid.ny);
   for (int sweep = 0; sweep < 2 && st == EM_OK; ++sweep) {
     for (int par = 0; par < 9 && st == EM_OK; ++par) {

@@ -87,6 +87,7 @@ struct FftCfg;
   template <>                                                                       \
   struct FftCfg<N_, V_> {                                                           \
     static constexpr int T = T_;                                                    \
+    static constexpr bool kTan = false;                                             \
     using Tw = TwChain<N_, T_, 1, __VA_ARGS__>;                                     \
     using TwTab = TwChainTab<N_, T_, 1, __VA_ARGS__>;                               \
     template <int DIR, int SYNC, class Chain>                                       \
@@ -120,12 +121,113 @@ EM_FFT_CFG(2048, 1, 128, 16, 8, 16)
 EM_FFT_CFG(2048, 2, 128, 8, 16, 16)
 #undef EM_FFT_CFG
 
+// Variant 3 at 1024: each pass's default sequence with its per-lane twiddles
+// in the factored FMA form (tan_u / radix4_lane / radix16_lane), from tables
+// built in FP64 per plan (tangent_tables).  Rows 16.16.4 read theirs from the
+// workgroup's LDS copy; columns 16.4.16 keep theirs in 18 registers, as many
+// as the complex twiddles they replace.  10% fewer VALU per line.
+//
+// Row table (2,688 floats): stage 2 (radix 16, 16 butterfly classes k) 16
+// entries of 40 floats (radix16_lane), then stage 3 (radix 4, k = 0..255)
+// 256 entries of 8: T1, T2, T3, g1, g2, g3/g1, 0, 0.
+struct TanRow1024 {
+  static constexpr bool ok = true;
+  static constexpr int F16 = 40 * 16, floats = F16 + 8 * 256;
+  static constexpr int entries = floats / 2;  // float2 slots of LDS
+  const float4* c16;
+  const float4* c4;
+  __device__ __forceinline__ static void fill(float2* tab, const float2* __restrict__ src, int tid, int nt) {
+    const float4* s = reinterpret_cast<const float4*>(src);
+    float4* d = reinterpret_cast<float4*>(tab);
+    for (int i = tid; i < floats / 4; i += nt) d[i] = s[i];
+  }
+  __device__ __forceinline__ void init(int tau, const float2* tab) {
+    const float4* t4 = reinterpret_cast<const float4*>(tab);
+    c16 = t4 + (tau % 16) * 10;
+    c4 = t4 + F16 / 4 + tau * 2;
+  }
+  template <int DIR, int SYNC>
+  __device__ __forceinline__ void run(float2 (&v)[16], float2* lds, int tau) const {
+    using L = FftLine<1024, 64, DIR, SYNC>;
+    L::template butterflies<16, 0>(v);
+    L::template store_stage<16, 1>(v, lds, tau);
+    lds_sync<SYNC>();
+    L::load_strided(v, lds, tau);
+    lds_sync<SYNC>();
+    radix16_lane<DIR, 0, 1, 16>(v, c16);
+    L::template store_stage<16, 16>(v, lds, tau);
+    lds_sync<SYNC>();
+    L::load_strided(v, lds, tau);
+    lds_sync<SYNC>();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      // butterfly j = tau + 64 u, k = j (256 classes)
+      const float4 e0 = c4[128 * u], e1 = c4[128 * u + 1];
+      radix4_lane<DIR>(v[u], v[u + 4], v[u + 8], v[u + 12], e0, make_float2(e1.x, e1.y));
+    }
+  }
+};
+
+// Column table (896 floats): stage 2 (radix 4, k = 0..15) 16 entries of 8 as
+// above; stage 3 (radix 16, k = 0..63) 64 entries of 12: lo_1..lo_3 (complex,
+// forward W^(n k)), then T1, T2, T3, g1, g2, g3/g1 of W^(4 k).
+struct TanCol1024 {
+  static constexpr bool in_registers = true;
+  float4 a0;
+  float2 a1;
+  float2 lo[3];
+  float4 h0;
+  float2 h1;
+  __device__ __forceinline__ void init(int tau, const float2* __restrict__ src) {
+    const float4* s = reinterpret_cast<const float4*>(src);
+    const float4 b0 = s[(tau % 16) * 2], b1 = s[(tau % 16) * 2 + 1];
+    a0 = b0;
+    a1 = make_float2(b1.x, b1.y);
+    const float4* t = s + 32 + tau * 3;
+    const float4 l01 = t[0], l2h = t[1], h = t[2];
+    lo[0] = make_float2(l01.x, l01.y);
+    lo[1] = make_float2(l01.z, l01.w);
+    lo[2] = make_float2(l2h.x, l2h.y);
+    h0 = make_float4(l2h.z, l2h.w, h.x, h.y);
+    h1 = make_float2(h.z, h.w);
+  }
+  template <int DIR, int SYNC>
+  __device__ __forceinline__ void run(float2 (&v)[16], float2* lds, int tau) const {
+    using L = FftLine<1024, 64, DIR, SYNC>;
+    L::template butterflies<16, 0>(v);
+    L::template store_stage<16, 1>(v, lds, tau);
+    lds_sync<SYNC>();
+    L::load_strided(v, lds, tau);
+    lds_sync<SYNC>();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) radix4_lane<DIR>(v[u], v[u + 4], v[u + 8], v[u + 12], a0, a1);
+    L::template store_stage<4, 16>(v, lds, tau);
+    lds_sync<SYNC>();
+    L::load_strided(v, lds, tau);
+    lds_sync<SYNC>();
+    radix16_lohi<DIR, 0, 1, 16>(v, lo, h0, h1);
+  }
+};
+
+template <>
+struct FftCfg<1024, 3> {
+  static constexpr int T = 64;
+  static constexpr bool kTan = true;
+  using Tw = TanCol1024;
+  using TwTab = TanRow1024;
+  template <int DIR, int SYNC, class Chain>
+  __device__ __forceinline__ static void run(float2 (&v)[16], float2* lds, int tau, const Chain& tws,
+                                             const float2*) {
+    tws.template run<DIR, SYNC>(v, lds, tau);
+  }
+};
+
 // Radix sequences of a size: three stages whose radices divide the values
 // per lane (16) at 1024 and 2048; one elsewhere (512: only 8.8.8 has three
 // stages; 768 needs its factor 3 in a radix dividing 12).
 template <int N>
 struct FftVariants {
-  static constexpr int count = (N == 1024 || N == 2048) ? 3 : 1;
+  static constexpr int count = N == 1024 ? 4 : (N == 2048 ? 3 : 1);
 };
 
 constexpr int kRowThreads = 256;
@@ -244,7 +346,7 @@ void row_pass_kernel(RowArgs a) {
   std::conditional_t<TWTAB, typename FftCfg<NX, V>::TwTab, typename FftCfg<NX, V>::Tw> tws;
   if constexpr (TWTAB) {
     float2* tab = smem + LINES * LineLds<NX>::size;
-    FftCfg<NX, V>::TwTab::fill(tab, a.tw, threadIdx.x, kRowThreads);
+    FftCfg<NX, V>::TwTab::fill(tab, FftCfg<NX, V>::kTan ? a.twt : a.tw, threadIdx.x, kRowThreads);
     __syncthreads();
     tws.init(tau, tab);
   } else {
@@ -435,7 +537,7 @@ void col_pass_kernel(ColArgs a) {
     __syncthreads();
     tws.init(tau, tab);
   } else {
-    tws.init(tau, a.tw);
+    tws.init(tau, FftCfg<NY, V>::kTan ? a.twt : a.tw);
   }
 
   constexpr bool LATE_P = ColLateP<NY>::value && (MODE & (COL_MUL_P | COL_CONJ_P)) != 0;
@@ -995,6 +1097,12 @@ static hipError_t row_dispatch(unsigned mode, const RowArgs& a, hipStream_t s) {
         return launch_pass(row_pass_kernel<NX, HOT, ZK, 1>, grid, dim3(kRowThreads), row_lds<NX, 1>(), s, a);
       if (mode == HOT && a.radix == 2)
         return launch_pass(row_pass_kernel<NX, HOT, ZK, 2>, grid, dim3(kRowThreads), row_lds<NX, 2>(), s, a);
+      if constexpr (FftVariants<NX>::count > 3) {
+        if (mode == HOT && a.radix == 3) {
+          if (!a.twt) return hipErrorInvalidValue;
+          return launch_pass(row_pass_kernel<NX, HOT, ZK, 3>, grid, dim3(kRowThreads), row_lds<NX, 3>(), s, a);
+        }
+      }
     }
     switch (mode) {
 #define EM_ROW_PRUNED_CASE(M) \
@@ -1084,6 +1192,12 @@ static hipError_t col_attrs(void) {
         e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, HOT, ZK, 2>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)ColCfg<NY, DefaultCols<NY>::value, 2>::LDS);
+      if constexpr (FftVariants<NY>::count > 3) {
+        if (e == hipSuccess)
+          e = hipFuncSetAttribute((const void*)col_pass_kernel<NY, DefaultCols<NY>::value, HOT, ZK, 3>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)ColCfg<NY, DefaultCols<NY>::value, 3>::LDS);
+      }
     }
   }
   if constexpr (FftCfg<NY>::T == 64) {
@@ -1159,6 +1273,12 @@ static hipError_t col_dispatch(unsigned mode, const ColArgs& a, hipStream_t s) {
       constexpr unsigned HOT = COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE;
       if (mode == HOT && !alt && a.radix == 1) return col_launch_one<NY, DefaultCols<NY>::value, HOT, ZK, 1>(a, s);
       if (mode == HOT && !alt && a.radix == 2) return col_launch_one<NY, DefaultCols<NY>::value, HOT, ZK, 2>(a, s);
+      if constexpr (FftVariants<NY>::count > 3) {
+        if (mode == HOT && !alt && a.radix == 3) {
+          if (!a.twt) return hipErrorInvalidValue;
+          return col_launch_one<NY, DefaultCols<NY>::value, HOT, ZK, 3>(a, s);
+        }
+      }
     }
     switch (mode) {
 #define EM_COL_PR_CASE(M)                                                                   \

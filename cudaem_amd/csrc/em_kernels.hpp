@@ -70,6 +70,7 @@ struct RowArgs {
   const float* V;           // [ny][nx]
   const float2* t;          // [ny][nx]
   const float2* tw;         // [nx] exp(-2 pi i m / nx)
+  const float2* twt;        // radix variant 3: the per-lane twiddle table (tangent_tables)
   float2* ckpt;             // [nprobe][ny][nx]
   float2* gpart;            // [gridDim.y][ny][nx] gradient partials
   float* o1;                // [nprobe][ny][nx] intensity or real part
@@ -114,6 +115,7 @@ struct ColArgs {
   int nx, ny, nprobe, pb;
   const float2* P;          // [ny][nx]
   const float2* tw;         // [ny]
+  const float2* twt;        // radix variant 3: the per-lane twiddle table (tangent_tables)
   const float2* A;          // [ny][nx]
   const float2* ex;         // [nprobe][nx]
   const float2* ey;         // [nprobe][ny]

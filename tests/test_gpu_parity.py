@@ -840,9 +840,9 @@ def test_band_skip_is_below_fp32_resolution(em):
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_radix_variants_agree(em, n):
     """The autotuner's radix sequences of the row and column lines (1024:
-    16.16.4, 16.4.16, 4.16.16; 2048: 16.16.8, 16.8.16, 8.16.16) give the
-    default sequence's exit waves to FP32 rounding, and the default matches
-    the oracle."""
+    16.16.4, 16.4.16, 4.16.16 and the per-lane factored-twiddle forms; 2048:
+    16.16.8, 16.8.16, 8.16.16) give the default sequence's exit waves to FP32
+    rounding, and the default matches the oracle."""
     import torch
     rng = np.random.default_rng(41)
     S, B, L = 4, 3, 20.0 * n / 1024
@@ -852,8 +852,9 @@ def test_radix_variants_agree(em, n):
         assert p.set_band_skip(1e-9)[0] > 0
         psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, L, B), rng.uniform(0, L, B))
         outs = {}
-        for rr in range(3):
-            for cr in range(3):
+        nv = 4 if n == 1024 else 3
+        for rr in range(nv):
+            for cr in range(nv):
                 p.set_tuning(row_radix=rr, col_radix=cr)
                 psi = psi0.clone()
                 p.multislice_forward(psi, t=tt)
