@@ -37,11 +37,13 @@ HBM_PEAK = 8.0e12
 # HBM traffic of the fused passes at 1024^2 per probe and launch, from
 # rocprofv3 --pmc FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000) with factors
 # from a 1 GiB device copy (profiles/batched_stem_1024_hbm_traffic.md), and
-# the launch shapes they were measured at.  The bench reports it only when its
-# own tuning is the same: other shapes move other bytes.
+# the launch shapes they were measured at (the plan defaults at 1024^2).  The
+# bench reports it only when its own tuning is the same: other shapes move
+# other bytes.
 PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
-              "persistent": 0, "row_radix": 0, "col_radix": 0, "probe_group": 37}
-PMC_BYTES_PER_PROBE = {"column pass": (254.85 + 228.52) / 37 * 2**20, "row pass": (230.20 + 222.00) / 37 * 2**20}
+              "persistent": 0, "row_radix": 0, "col_radix": 1, "probe_group": 37}
+PMC_BYTES_PER_PROBE = {"column pass": (132363 * 2 + 236509) * 1024 / 37,
+                       "row pass": (116548 * 2 + 224695) * 1024 / (256 / 7)}
 T_START = time.perf_counter()
 
 
