@@ -870,6 +870,38 @@ def test_radix_variants_agree(em, n):
     assert e < TOL_WAVE
 
 
+def test_radix_variant3_on_a_non_square_grid(em):
+    """Radix variant 3 (per-lane twiddles in factored FMA form) exists for
+    1024-point lines only: on a 1024 x 512 grid the rows take it and give the
+    default sequence's exit waves to FP32 rounding, while the 512-point
+    columns, and the lines of a 2048^2 plan, refuse it."""
+    import torch
+    from cudaem_amd._capi import EmError
+    rng = np.random.default_rng(43)
+    nx, ny, S, B = 1024, 512, 4, 2
+    lx, ly = 20.0, 10.0
+    t = np.exp(1j * rng.uniform(0, 0.5, (S, ny, nx)))
+    tt = torch.from_numpy(t.astype(np.complex64)).cuda()
+    with em.Plan(nx, ny, lx, ly, 2.0, 300.0, max_batch=B) as p:
+        assert p.set_band_skip(1e-9)[0] > 0
+        psi0 = p.build_probes(probe_lens(20.0), rng.uniform(0, lx, B), rng.uniform(0, ly, B))
+        outs = []
+        for rr in (0, 3):
+            p.set_tuning(row_radix=rr)
+            psi = psi0.clone()
+            p.multislice_forward(psi, t=tt)
+            outs.append(_np(psi))
+        with pytest.raises(EmError):
+            p.set_tuning(col_radix=3)
+        torch.cuda.synchronize()
+    assert np.abs(outs[0]).max() > 0
+    assert rel_l2(outs[1], outs[0]) < 2e-6
+    with em.Plan(2048, 2048, 40.0, 40.0, 2.0, 300.0) as p:
+        for kw in (dict(row_radix=3), dict(col_radix=3)):
+            with pytest.raises(EmError):
+                p.set_tuning(**kw)
+
+
 def test_persistent_slice_loop_matches_passes(em):
     """The persistent slice loop (one cooperative launch per probe group for
     the interior passes) gives the separate passes' results: the forward loop
