@@ -132,7 +132,6 @@ struct Dft<4, DIR, B, S, E> {
   }
 };
 
-// Composite R = R1*R2 by the four-step split n = R2*n1 + n2, k = k1 + R1*k2.
 // A compile-time twiddle w = exp(DIR 2 pi i M / N) in the factored form
 // w = g (1 + i tau) (g = cos, tau = tan of its angle), so that w x = g u with
 // u = x + i tau x: two FMAs (two adds when tau = +-1), and the real g can
@@ -249,8 +248,12 @@ __device__ __forceinline__ void radix4_lane(float2& x0, float2& x1, float2& x2, 
   radix4_scaled<DIR>(x0, x1, x2, x3, u1, u2, u3, c0.w, c1.x, c1.y);
 }
 
-template <int R1, int R2, int DIR, int B, int S, int E>
+// Composite R = R1 * 4 by the four-step split n = 4 n1 + n2, k = k1 + R1 k2:
+// R1-point DFTs over n1 for each n2, then per k1 the radix-4 over n2 with the
+// twiddles W_R^(n2 k1) folded into its FMAs (Radix4Tw).
+template <int R1, int DIR, int B, int S, int E>
 struct DftComposite {
+  static constexpr int R2 = 4;
   static constexpr int R = R1 * R2;
   template <int N2>
   __device__ __forceinline__ static void step1(float2 (&v)[E]) {
@@ -259,7 +262,6 @@ struct DftComposite {
       step1<N2 + 1>(v);
     }
   }
-  // R2 = 4: the twiddles of step 2 folded into step 3's radix-4 butterflies
   template <int K1>
   __device__ __forceinline__ static void step23(float2 (&v)[E]) {
     if constexpr (K1 < R1) {
@@ -267,33 +269,9 @@ struct DftComposite {
       step23<K1 + 1>(v);
     }
   }
-  template <int N2, int K1>
-  __device__ __forceinline__ static void step2(float2 (&v)[E]) {
-    if constexpr (N2 < R2) {
-      if constexpr (K1 < R1) {
-        constexpr int idx = B + N2 * S + K1 * R2 * S;
-        v[idx] = ctw<R, N2 * K1, DIR>(v[idx]);
-        step2<N2, K1 + 1>(v);
-      } else {
-        step2<N2 + 1, 0>(v);
-      }
-    }
-  }
-  template <int K1>
-  __device__ __forceinline__ static void step3(float2 (&v)[E]) {
-    if constexpr (K1 < R1) {
-      Dft<R2, DIR, B + K1 * R2 * S, S, E>::run(v);
-      step3<K1 + 1>(v);
-    }
-  }
   __device__ __forceinline__ static void run(float2 (&v)[E]) {
     step1<0>(v);
-    if constexpr (R2 == 4) {
-      step23<0>(v);
-    } else {
-      step2<0, 0>(v);
-      step3<0>(v);
-    }
+    step23<0>(v);
     // X[k1 + R1 k2] sits at slot R2 k1 + k2: reorder to natural order
     float2 tmp[R];
 #pragma unroll
@@ -307,23 +285,15 @@ struct DftComposite {
 
 template <int DIR, int B, int S, int E>
 struct Dft<8, DIR, B, S, E> {
-  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<2, 4, DIR, B, S, E>::run(v); }
-};
-template <int DIR, int B, int S, int E>
-struct Dft<16, DIR, B, S, E> {
-  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<4, 4, DIR, B, S, E>::run(v); }
-};
-template <int DIR, int B, int S, int E>
-struct Dft<6, DIR, B, S, E> {
-  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<2, 3, DIR, B, S, E>::run(v); }
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<2, DIR, B, S, E>::run(v); }
 };
 template <int DIR, int B, int S, int E>
 struct Dft<12, DIR, B, S, E> {
-  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<3, 4, DIR, B, S, E>::run(v); }
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<3, DIR, B, S, E>::run(v); }
 };
 template <int DIR, int B, int S, int E>
-struct Dft<32, DIR, B, S, E> {
-  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<4, 8, DIR, B, S, E>::run(v); }
+struct Dft<16, DIR, B, S, E> {
+  __device__ __forceinline__ static void run(float2 (&v)[E]) { DftComposite<4, DIR, B, S, E>::run(v); }
 };
 
 // --------------------------------------------------------------------------
