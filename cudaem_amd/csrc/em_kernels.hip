@@ -119,6 +119,7 @@ EM_FFT_CFG(1024, 2, 64, 4, 16, 16)
 EM_FFT_CFG(2048, 0, 128, 16, 16, 8)
 EM_FFT_CFG(2048, 1, 128, 16, 8, 16)
 EM_FFT_CFG(2048, 2, 128, 8, 16, 16)
+EM_FFT_CFG(4096, 0, 256, 16, 16, 16)
 #undef EM_FFT_CFG
 
 // Variant 3 at 1024: each pass's default sequence with its per-lane twiddles
@@ -257,7 +258,7 @@ int pruned_slots(int n, int lo, int hi) {
   case N_:               \
     T = FftCfg<N_>::T;   \
     break;
-    EM_T(64) EM_T(128) EM_T(256) EM_T(512) EM_T(768) EM_T(1024) EM_T(2048)
+    EM_T(64) EM_T(128) EM_T(256) EM_T(512) EM_T(768) EM_T(1024) EM_T(2048) EM_T(4096)
 #undef EM_T
     default:
       return 0;
@@ -273,14 +274,15 @@ int pruned_slots(int n, int lo, int hi) {
 // the alternative the autotuner may pick for the per-slice modes. Eight
 // columns (a 64 KiB + pad LDS line buffer) let two workgroups share a CU, so
 // one loads while the other transforms; XCD-banded tile order keeps the
-// half-segment reads that costs from being fetched twice.
+// half-segment reads that costs from being fetched twice.  4096-point
+// columns take 256 threads each, so four fill a 1024-thread workgroup.
 template <int N>
 struct DefaultCols {
-  static constexpr int value = 8;
+  static constexpr int value = FftCfg<N>::T >= 256 ? 4 : 8;
 };
 template <int N>
 struct AltCols {
-  static constexpr int value = (FftCfg<N>::T >= 128) ? 4 : 16;
+  static constexpr int value = FftCfg<N>::T >= 256 ? 2 : ((FftCfg<N>::T >= 128) ? 4 : 16);
 };
 template <int N, int COLS_ = DefaultCols<N>::value, int V = 0>
 struct ColCfg {
@@ -313,6 +315,17 @@ struct RowOcc {
   static constexpr int waves = (MODE & ROW_T_STREAM) ? 4 : 1;
 };
 
+// The row pass keeps its twiddles in an LDS table after its line buffers
+// where both fit the default 64 KiB of dynamic LDS (every size up to 2048);
+// 4096-point rows (one line, 34 KiB, per workgroup; the table would take 32
+// KiB more) form them in registers.
+template <int NX, int V = 0>
+struct RowTab {
+  static constexpr int LINES = kRowThreads / FftCfg<NX, V>::T;
+  static constexpr bool use = FftCfg<NX, V>::TwTab::ok &&
+      ((size_t)LINES * LineLds<NX>::size + FftCfg<NX, V>::TwTab::entries) * sizeof(float2) <= 65536;
+};
+
 // ZK > 0: the pruned form of a slice-loop pass (skip_in and skip_out set).
 template <int NX, unsigned MODE, int ZK = 0, int V = 0>
 __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(RowOcc<MODE>::waves)))
@@ -342,7 +355,7 @@ void row_pass_kernel(RowArgs a) {
   constexpr int SYNC = (T <= 64) ? SYNC_WAVE : SYNC_BLOCK;  // a line inside one wavefront
   // Twiddles: a per-workgroup LDS table after the line buffers where every
   // stage allows it (one read per twiddle instead of forming it in registers)
-  constexpr bool TWTAB = FftCfg<NX, V>::TwTab::ok;
+  constexpr bool TWTAB = RowTab<NX, V>::use;
   std::conditional_t<TWTAB, typename FftCfg<NX, V>::TwTab, typename FftCfg<NX, V>::Tw> tws;
   if constexpr (TWTAB) {
     float2* tab = smem + LINES * LineLds<NX>::size;
@@ -476,14 +489,14 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
 }
 
 // Whether the column pass reads P right before the multiply instead of
-// holding it in registers through the forward FFT.  Held, it makes the 768
-// and 2048 kernels spill at 4 waves per SIMD (16 B and 148 B of scratch);
+// holding it in registers through the forward FFT.  Held, it makes the 768,
+// 2048 and 4096 kernels spill at 4 waves per SIMD (16 B, 148 B and 152 B);
 // read late they spill 0 B and 20 B, and the 2048 pass runs 24% faster per
 // probe (26.0 against 34.0 us, full width).  At 512 and 1024 the held P is as
 // fast or faster (1024: 4.0 against 5.05 us per probe).
 template <int NY>
 struct ColLateP {
-  static constexpr bool value = NY == 768 || NY == 2048;
+  static constexpr bool value = NY == 768 || NY >= 2048;
 };
 
 // At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
@@ -1051,7 +1064,7 @@ hipError_t launch_persist(const PersistArgs& args, int grid, hipStream_t s) {
 #define EM_COL_PRUNED_CHUNK_MODES(X)                                            \
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS)
 
-#define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048)
+#define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048) X(4096)
 
 int fft_variants(int n) {
   switch (n) {
@@ -1080,7 +1093,7 @@ bool fft_size_supported(int n) {
 template <int NX, int V = 0>
 constexpr size_t row_lds() {
   constexpr int LINES = kRowThreads / FftCfg<NX>::T;
-  constexpr int TAB = FftCfg<NX, V>::TwTab::ok ? FftCfg<NX, V>::TwTab::entries : 0;
+  constexpr int TAB = RowTab<NX, V>::use ? FftCfg<NX, V>::TwTab::entries : 0;
   return ((size_t)LINES * LineLds<NX>::size + TAB) * sizeof(float2);
 }
 

@@ -27,7 +27,7 @@ def em(cuda):
     return cudaem_amd
 
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 2048])
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 2048, 4096])
 def test_fft2d_matches_oracle(em, n):
     import torch
     rng = np.random.default_rng(n)
@@ -302,12 +302,15 @@ def test_stem_probes_exit_waves_and_detectors(em):
     assert rel_l2(stem, ref_det) < TOL_INT
 
 
-@pytest.mark.parametrize("n", [768, 2048])
+@pytest.mark.parametrize("n", [768, 2048, 4096])
 def test_probe_multislice_large_grids(em, n):
     """Probe exit waves and detector sums at 768^2 (radix-12 stages with LDS
-    twiddle tables) and 2048^2 (two wavefronts per column line); both read
-    the propagator late in the column pass.  A thin SrTiO3 block keeps the
-    oracle's FP64 FFTs short."""
+    twiddle tables), 2048^2 (two wavefronts per column line) and 4096^2 (four
+    wavefronts per line, 4-column tiles, row twiddles in registers); all read
+    the propagator late in the column pass.  The batched STEM call (probe
+    build, pruned slice loop and last-slice detector pass, graph-launched)
+    gives the same detector sums.  A thin SrTiO3 block keeps the oracle's
+    FP64 FFTs short."""
     import torch
     import cudaem_amd.specimen as sp
     at = sp.srtio3(3, 3, 2)
@@ -324,6 +327,7 @@ def test_probe_multislice_large_grids(em, n):
         p.multislice_forward(psi, V=V)
         det = _np(p.detector_sums(psi, inner, outer))
         Psi = _np(psi)
+        stem = _np(p.stem(lens, xs, ys, inner, outer))
         torch.cuda.synchronize()
     o = OracleSpecimen(at, n, n, at["lx"], at["ly"], dz, e0, O.BORDER_MIN)
     assert S == o.S
@@ -340,6 +344,8 @@ def test_probe_multislice_large_grids(em, n):
         for d in range(len(inner)):
             ref[d, b] = O.detector_sum(Psi_o, at["lx"], at["ly"], g2[0][d], g2[1][d])
     assert rel_l2(det, ref) < TOL_INT
+    print(f"n={n} detector relL2 {rel_l2(det, ref):.3e}, batched STEM {rel_l2(stem, ref):.3e}")
+    assert rel_l2(stem, ref) < TOL_INT
 
 
 def test_non_square_grid_exit_waves_and_stem(em):
