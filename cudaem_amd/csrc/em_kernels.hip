@@ -8,7 +8,7 @@
 // the wave once.  Between passes the wave is kept in (y, kx) layout, row-major.
 // Both passes reuse the transmission / propagator values held in registers
 // for every probe a workgroup processes (the column pass reads P at the
-// multiply instead where holding it would spill: 768, 2048).  Chunk-table
+// multiply instead where holding it would spill: 768, 2048, 4096).  Chunk-table
 // modes give each workgroup row its own wave and table (ragged batches, a
 // potential stack per wave).
 #include <hip/hip_ext.h>
