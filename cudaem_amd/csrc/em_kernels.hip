@@ -111,14 +111,17 @@ struct FftCfg;
 EM_FFT_CFG(64, 0, 8, 8, 8)
 EM_FFT_CFG(128, 0, 16, 8, 8, 2)
 EM_FFT_CFG(256, 0, 16, 16, 16)
+EM_FFT_CFG(384, 0, 32, 12, 4, 4, 2)
 EM_FFT_CFG(512, 0, 64, 8, 8, 8)
 EM_FFT_CFG(768, 0, 64, 12, 4, 4, 4)
 EM_FFT_CFG(1024, 0, 64, 16, 16, 4)
 EM_FFT_CFG(1024, 1, 64, 16, 4, 16)
 EM_FFT_CFG(1024, 2, 64, 4, 16, 16)
+EM_FFT_CFG(1536, 0, 128, 12, 4, 4, 4, 2)
 EM_FFT_CFG(2048, 0, 128, 16, 16, 8)
 EM_FFT_CFG(2048, 1, 128, 16, 8, 16)
 EM_FFT_CFG(2048, 2, 128, 8, 16, 16)
+EM_FFT_CFG(3072, 0, 256, 12, 4, 4, 4, 4)
 EM_FFT_CFG(4096, 0, 256, 16, 16, 16)
 #undef EM_FFT_CFG
 
@@ -225,7 +228,11 @@ struct FftCfg<1024, 3> {
 
 // Radix sequences of a size: three stages whose radices divide the values
 // per lane (16) at 1024 and 2048; one elsewhere (512: only 8.8.8 has three
-// stages; 768 needs its factor 3 in a radix dividing 12).
+// stages; the 3 * 2^k lengths need their factor 3 in a radix dividing the 12
+// values per lane, which leaves radices of at most 4 for the rest: 384 =
+// 12.4.4.2, 768 = 12.4.4.4, 1536 = 12.4.4.4.2, 3072 = 12.4.4.4.4).  24
+// values per lane (radix 24 = 6 x 4, three or four stages) spilled 300-500 B
+// per column kernel at the 128-VGPR budget.
 template <int N>
 struct FftVariants {
   static constexpr int count = N == 1024 ? 4 : (N == 2048 ? 3 : 1);
@@ -258,7 +265,8 @@ int pruned_slots(int n, int lo, int hi) {
   case N_:               \
     T = FftCfg<N_>::T;   \
     break;
-    EM_T(64) EM_T(128) EM_T(256) EM_T(512) EM_T(768) EM_T(1024) EM_T(2048) EM_T(4096)
+    EM_T(64) EM_T(128) EM_T(256) EM_T(384) EM_T(512) EM_T(768) EM_T(1024) EM_T(1536)
+    EM_T(2048) EM_T(3072) EM_T(4096)
 #undef EM_T
     default:
       return 0;
@@ -274,8 +282,9 @@ int pruned_slots(int n, int lo, int hi) {
 // the alternative the autotuner may pick for the per-slice modes. Eight
 // columns (a 64 KiB + pad LDS line buffer) let two workgroups share a CU, so
 // one loads while the other transforms; XCD-banded tile order keeps the
-// half-segment reads that costs from being fetched twice.  4096-point
-// columns take 256 threads each, so four fill a 1024-thread workgroup.
+// half-segment reads that costs from being fetched twice.  3072- and
+// 4096-point columns take 256 threads each, so four fill a 1024-thread
+// workgroup.
 template <int N>
 struct DefaultCols {
   static constexpr int value = FftCfg<N>::T >= 256 ? 4 : 8;
@@ -492,11 +501,13 @@ __device__ __forceinline__ double band_g2(int ix, int iy, const GridDev& g) {
 // holding it in registers through the forward FFT.  Held, it makes the 768,
 // 2048 and 4096 kernels spill at 4 waves per SIMD (16 B, 148 B and 152 B);
 // read late they spill 0 B and 20 B, and the 2048 pass runs 24% faster per
-// probe (26.0 against 34.0 us, full width).  At 512 and 1024 the held P is as
-// fast or faster (1024: 4.0 against 5.05 us per probe).
+// probe (26.0 against 34.0 us, full width).  The other 3 * 2^k lengths (384,
+// 1536, 3072) spill 88 B, 24 B and 12 B with P held, 12 B, 28 B and 12 B
+// with P read late.  At 512 and 1024 the held P is as fast or faster (1024:
+// 4.0 against 5.05 us per probe).
 template <int NY>
 struct ColLateP {
-  static constexpr bool value = NY == 768 || NY >= 2048;
+  static constexpr bool value = NY % 3 == 0 || NY >= 2048;
 };
 
 // At most 128 VGPRs (4 waves per SIMD): two 512-thread workgroups of the
@@ -1064,7 +1075,8 @@ hipError_t launch_persist(const PersistArgs& args, int grid, hipStream_t s) {
 #define EM_COL_PRUNED_CHUNK_MODES(X)                                            \
   X(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS)
 
-#define EM_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(2048) X(4096)
+#define EM_SIZES(X) \
+  X(64) X(128) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
 
 int fft_variants(int n) {
   switch (n) {

@@ -45,9 +45,9 @@ def test_lens_layout_is_shared():
 
 
 def test_fft_sizes(C):
-    for n in (64, 128, 256, 512, 768, 1024, 2048, 4096):
+    for n in (64, 128, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096):
         assert C.lib().em_fft_size_supported(n) == 1
-    for n in (0, 100, 1000, 1536, 8192):
+    for n in (0, 100, 192, 1000, 1280, 6144, 8192):
         assert C.lib().em_fft_size_supported(n) == 0
 
 

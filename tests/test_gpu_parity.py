@@ -27,7 +27,7 @@ def em(cuda):
     return cudaem_amd
 
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 2048, 4096])
+@pytest.mark.parametrize("n", [64, 128, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096])
 def test_fft2d_matches_oracle(em, n):
     import torch
     rng = np.random.default_rng(n)
@@ -48,10 +48,11 @@ def test_fft2d_matches_oracle(em, n):
         assert rel_l2(fwd[0], O.fft2d(x[0])) < TOL_OP
 
 
-def test_fft2d_non_square(em):
+@pytest.mark.parametrize("nx,ny", [(256, 128), (384, 1536), (3072, 512)])
+def test_fft2d_non_square(em, nx, ny):
     import torch
     rng = np.random.default_rng(7)
-    nx, ny = 256, 128
+    print(f"fft2d nx={nx} ny={ny}", flush=True)
     x = (rng.standard_normal((3, ny, nx)) + 1j * rng.standard_normal((3, ny, nx))).astype(np.complex64)
     with em.Plan(nx, ny, 20.0, 10.0, 1.0, 200.0, max_batch=3) as p:
         d = torch.from_numpy(x).cuda()
@@ -302,12 +303,13 @@ def test_stem_probes_exit_waves_and_detectors(em):
     assert rel_l2(stem, ref_det) < TOL_INT
 
 
-@pytest.mark.parametrize("n", [768, 2048, 4096])
+@pytest.mark.parametrize("n", [384, 768, 1536, 2048, 3072, 4096])
 def test_probe_multislice_large_grids(em, n):
-    """Probe exit waves and detector sums at 768^2 (radix-12 stages with LDS
-    twiddle tables), 2048^2 (two wavefronts per column line) and 4096^2 (four
-    wavefronts per line, 4-column tiles, row twiddles in registers); all read
-    the propagator late in the column pass.  The batched STEM call (probe
+    """Probe exit waves and detector sums at the 3 * 2^k sizes (384^2, 768^2,
+    1536^2, 3072^2: a radix-12 first stage, then radix 4 and 2 stages with
+    LDS twiddle tables), 2048^2 (two wavefronts per column line) and 4096^2
+    (four wavefronts per line, 4-column tiles, row twiddles in registers); all
+    read the propagator late in the column pass.  The batched STEM call (probe
     build, pruned slice loop and last-slice detector pass, graph-launched)
     gives the same detector sums.  A thin SrTiO3 block keeps the oracle's
     FP64 FFTs short."""
