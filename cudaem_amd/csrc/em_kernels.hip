@@ -76,6 +76,22 @@ __device__ __forceinline__ UniformChunk read_chunk(const PassChunk* c) {
   return u;
 }
 
+// A raw buffer over one probe's wave plane (stride 0; dword 3 = 32-bit data
+// format, the gfx9 setting).  Byte offsets at or past `bytes` are out of
+// range: a buffer load there returns 0 and a buffer store is dropped, with
+// no memory access.
+constexpr std::uint32_t kOutOfRange = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float2* p, std::uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(p), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 plane_load(__amdgpu_buffer_rsrc_t r, std::uint32_t off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void plane_store(float2 v, __amdgpu_buffer_rsrc_t r, std::uint32_t off) {
+  using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
+}
+
 // --------------------------------------------------------------------------
 // FFT configurations per line length: threads per line and radix sequence.
 // --------------------------------------------------------------------------
@@ -397,11 +413,34 @@ void row_pass_kernel(RowArgs a) {
     for (int e = 0; e < E; ++e) g[e] = make_float2(0.f, 0.f);
   }
 
+  // The pruned form (ZK > 0, launched only with skip_in and skip_out both
+  // set) reads and writes the wave through a buffer per probe plane, at byte
+  // offsets formed once here: a column in the band-limit skip gets an
+  // out-of-range offset, so its load reads 0 and its store is dropped without
+  // a branch per lane around each access.
+  constexpr bool BUF = ZK > 0;
+  std::uint32_t off[BUF ? E : 1];
+  if constexpr (BUF) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kx = tau + T * e;
+      const bool skip = kx >= a.skip_lo && kx <= a.skip_hi;
+      off[e] = (std::uint32_t)((row + kx) * (long long)sizeof(float2)) | (skip ? kOutOfRange : 0u);
+    }
+  }
+  const std::uint32_t plane_bytes = (std::uint32_t)(a.pstride * (long long)sizeof(float2));
+
   // Short lines (8 values per lane): the next probe's row is loaded while
   // this probe is transformed (as in the column pass).
   constexpr bool PREFETCH = (E <= 8) && (MODE & ROW_IN_PLANE) == 0;
   float2 nv[PREFETCH ? E : 1];
-  auto load_row = [&](long long rbase, float2* dst) {
+  auto load_row = [&](long long pbase, float2* dst) {
+    if constexpr (BUF) {
+      const __amdgpu_buffer_rsrc_t r = plane_rsrc(in + pbase, plane_bytes);
+#pragma unroll
+      for (int e = 0; e < E; ++e) dst[e] = zero_slot<E, ZK>(e) ? make_float2(0.f, 0.f) : plane_load(r, off[e]);
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kx = tau + T * e;
@@ -409,12 +448,12 @@ void row_pass_kernel(RowArgs a) {
         dst[e] = make_float2(0.f, 0.f);
       } else {
         const bool skip = a.skip_in && kx >= a.skip_lo && kx <= a.skip_hi;
-        dst[e] = skip ? make_float2(0.f, 0.f) : in[rbase + kx];
+        dst[e] = skip ? make_float2(0.f, 0.f) : in[pbase + row + kx];
       }
     }
   };
   if constexpr (PREFETCH) {
-    if (b0 < b1) load_row((long long)b0 * a.pstride + row, nv);
+    if (b0 < b1) load_row((long long)b0 * a.pstride, nv);
   }
 
   for (int b = b0; b < b1; ++b) {
@@ -426,9 +465,9 @@ void row_pass_kernel(RowArgs a) {
     } else if constexpr (PREFETCH) {
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nv[e];
-      if (b + 1 < b1) load_row(base + a.pstride, nv);
+      if (b + 1 < b1) load_row((long long)(b + 1) * a.pstride, nv);
     } else {
-      load_row(base, v);
+      load_row((long long)b * a.pstride, v);
     }
     if constexpr ((MODE & ROW_PRE_IFFT) != 0) FftCfg<NX, V>::template run<+1, SYNC>(v, lds, tau, tws, a.tw);
     if constexpr ((MODE & ROW_CKPT_STORE) != 0) {
@@ -471,6 +510,11 @@ void row_pass_kernel(RowArgs a) {
           a.o2[i] = a.wgt * v[e].y;
         }
       }
+    } else if constexpr (BUF) {
+      const __amdgpu_buffer_rsrc_t r = plane_rsrc(out + (long long)b * a.pstride, plane_bytes);
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (!zero_slot<E, ZK>(e)) plane_store(v[e], r, off[e]);
     } else {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
