@@ -423,18 +423,28 @@ void row_pass_kernel(RowArgs a) {
   }
 
   // The pruned form (ZK > 0, launched only with skip_in and skip_out both
-  // set) reads and writes the wave through a buffer per probe plane, at byte
-  // offsets formed once here: a column in the band-limit skip gets an
-  // out-of-range offset, so its load reads 0 and its store is dropped without
-  // a branch per lane around each access.
-  constexpr bool BUF = ZK > 0;
+  // set) and the 8-value lines read and write the wave through a buffer per
+  // probe plane, at byte offsets formed once here: a column in the band-limit
+  // skip gets an out-of-range offset, so its load reads 0 and its store is
+  // dropped without a branch per lane around each access.  (Unpruned 12- and
+  // 16-value lines keep global addresses: the offsets of both directions
+  // took 30 more VGPRs at 384 and 768, a wave per SIMD fewer.)
+  constexpr bool BUF = ZK > 0 || E <= 8;
+  constexpr bool TWO = ZK == 0;
   std::uint32_t off[BUF ? E : 1];
+  std::uint32_t offo[(BUF && TWO) ? E : 1];
   if constexpr (BUF) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int kx = tau + T * e;
-      const bool skip = kx >= a.skip_lo && kx <= a.skip_hi;
-      off[e] = (std::uint32_t)((row + kx) * (long long)sizeof(float2)) | (skip ? kOutOfRange : 0u);
+      const bool band = kx >= a.skip_lo && kx <= a.skip_hi;
+      const std::uint32_t o = (std::uint32_t)((row + kx) * (long long)sizeof(float2));
+      if constexpr (TWO) {
+        off[e] = o | ((a.skip_in && band) ? kOutOfRange : 0u);
+        offo[e] = o | ((a.skip_out && band) ? kOutOfRange : 0u);
+      } else {
+        off[e] = o | (band ? kOutOfRange : 0u);
+      }
     }
   }
   const std::uint32_t plane_bytes = (std::uint32_t)(a.pstride * (long long)sizeof(float2));
@@ -523,7 +533,7 @@ void row_pass_kernel(RowArgs a) {
       const __amdgpu_buffer_rsrc_t r = plane_rsrc(out + (long long)b * a.pstride, plane_bytes);
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        if (!zero_slot<E, ZK>(e)) plane_store(v[e], r, off[e]);
+        if (!zero_slot<E, ZK>(e)) plane_store(v[e], r, TWO ? offo[e] : off[e]);
     } else {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
