@@ -41,7 +41,7 @@ HBM_PEAK = 8.0e12
 # bench reports it only when its own tuning is the same: other shapes move
 # other bytes.
 PMC_SHAPES = {"row_blocks": 768, "col_blocks": 4096, "col_tile": 0, "col_variant": 1, "row_variant": 0,
-              "persistent": 0, "row_radix": 0, "col_radix": 1, "probe_group": 37}
+              "persistent": 0, "row_radix": 1, "col_radix": 1, "probe_group": 37}
 PMC_BYTES_PER_PROBE = {"column pass": (132363 * 2 + 236509) * 1024 / 37,
                        "row pass": (116548 * 2 + 224695) * 1024 / (256 / 7)}
 T_START = time.perf_counter()
@@ -68,6 +68,10 @@ def parse():
                     help="row-pass workgroup target instead of the default (with --no-autotune)")
     ap.add_argument("--col-blocks", type=int, default=0,
                     help="column-pass workgroup target instead of the default (with --no-autotune)")
+    ap.add_argument("--row-radix", type=int, default=-1,
+                    help="row-pass radix sequence instead of the default (with --no-autotune)")
+    ap.add_argument("--col-radix", type=int, default=-1,
+                    help="column-pass radix sequence instead of the default (with --no-autotune)")
     ap.add_argument("--plain-probes", type=int, default=32,
                     help="probes of the one-probe-per-launch side of batched_vs_plain (each grid)")
     ap.add_argument("--persistent", action="store_true",
@@ -375,6 +379,10 @@ def main():
             plan.set_tuning(col_blocks=args.col_blocks)
         if args.persistent:
             plan.set_tuning(persistent=1)
+        if args.row_radix >= 0:
+            plan.set_tuning(row_radix=args.row_radix)
+        if args.col_radix >= 0:
+            plan.set_tuning(col_radix=args.col_radix)
     else:
         tuned, tune_ms = plan.autotune(min(args.batch, npos), nslices=8)
         if args.group:

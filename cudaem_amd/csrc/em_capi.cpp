@@ -681,15 +681,16 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
   EM_HIP(raw, hipDeviceSynchronize());
   compute_band_skip(raw);
   p->tune.row_blocks = whole_round_row_target(raw);
-  // radix sequences measured fastest untuned (batched STEM, A/B of all
-  // nine pairs): 16.4.16 columns at 1024 (2% faster column pass), 16.8.16
-  // rows at 2048 (1% faster row pass)
+  // radix sequences measured fastest untuned (batched STEM, A/B of the
+  // pairs): 16.4.16 columns and rows at 1024 (with the rows' buffer
+  // access, 16.4.16 rows are 2% faster than 16.16.4; the columns' two
+  // sequences tie), 16.8.16 rows at 2048 (1% faster row pass)
   p->tune.col_radix = grid->ny == 1024 ? 1 : 0;
   // two probes per column-pass workgroup at 1024 (P loaded once for both):
   // 3% faster column passes than one probe each (batched STEM, 256-probe
   // batches: 89.8 against 92.8 us per 37-probe launch, same box)
   if (grid->ny == 1024) p->tune.col_blocks = 4096;
-  p->tune.row_radix = grid->nx == 2048 ? 1 : 0;
+  p->tune.row_radix = (grid->nx == 1024 || grid->nx == 2048) ? 1 : 0;
   *out = p.release();
   return EM_OK;
 }

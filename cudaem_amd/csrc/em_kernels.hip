@@ -150,9 +150,9 @@ EM_FFT_CFG(3072, 0, 256, 12, 4, 4, 4, 4)
 EM_FFT_CFG(4096, 0, 256, 16, 16, 16)
 #undef EM_FFT_CFG
 
-// Variant 3 at 1024: each pass's default sequence with its per-lane twiddles
-// in the factored FMA form (tan_u / radix4_lane / radix16_lane), from tables
-// built in FP64 per plan (tangent_tables).  Rows 16.16.4 read theirs from the
+// Variant 3 at 1024: 16.16.4 rows and 16.4.16 columns with their per-lane
+// twiddles in the factored FMA form (tan_u / radix4_lane / radix16_lane), from
+// tables built in FP64 per plan (tangent_tables).  Rows 16.16.4 read theirs from the
 // workgroup's LDS copy; columns 16.4.16 keep theirs in 18 registers, as many
 // as the complex twiddles they replace.  10% fewer VALU per line.
 //
