@@ -845,6 +845,41 @@ def test_band_skip_is_below_fp32_resolution(em):
     assert rel_l2(res[1e-9][2], res[0.0][2]) < 1e-6
 
 
+@pytest.mark.parametrize("n,L,thr", [(512, 39.05, 1e-9), (512, 12.0, 1e-6), (1024, 20.0, 1e-6)])
+def test_skipped_columns_are_neither_read_nor_written(em, n, L, thr):
+    """The slice-loop row pass treats columns inside the band-limit skip as
+    zero without reading them and leaves them unwritten: with NaN planted in
+    those columns of the (y, kx) input, every other column after two row and
+    two column passes is bit-identical to the run with zeros there, and
+    finite.  Cases: 512 over 39.05 A (a skip band too narrow to prune a line
+    slot, so every skipped column takes an out-of-range buffer offset); 512
+    over 12 A and 1024 over 20 A at a threshold of 1e-6, whose bands reach
+    12-13 and 27-28 columns past each side of the pruned slots (those columns
+    take the out-of-range offsets)."""
+    import torch
+    rng = np.random.default_rng(47)
+    B = 2
+    with em.Plan(n, n, L, L, 1.9525, 300.0, max_batch=B) as p:
+        lo, hi = p.set_band_skip(thr)
+        assert 0 < lo <= hi < n
+        V = torch.from_numpy(rng.uniform(0.0, 0.5, (n, n)).astype(np.float32)).cuda()
+        w = (rng.standard_normal((B, n, n)) + 1j * rng.standard_normal((B, n, n))).astype(np.complex64)
+        outs = []
+        for fill in (np.nan, 0.0):
+            wf = w.copy()
+            wf[:, :, lo:hi + 1] = fill
+            wt = torch.from_numpy(wf).cuda()
+            p.time_passes(V, wt, iters=1)
+            torch.cuda.synchronize()
+            outs.append(_np(wt))
+    keep = np.ones(n, dtype=bool)
+    keep[lo:hi + 1] = False
+    a, b = (np.ascontiguousarray(o[:, :, keep]) for o in outs)
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.abs(b).max() > 0
+
+
 @pytest.mark.parametrize("n", [1024, 2048])
 def test_radix_variants_agree(em, n):
     """The autotuner's radix sequences of the row and column lines (1024:
