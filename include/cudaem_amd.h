@@ -119,10 +119,11 @@ em_status em_plan_pass_times(em_plan plan, double* row_ms, double* col_ms, int* 
  * (1024^2 grids with the band-limit skip on; the call then waits for its
  * work to finish, and reports a barrier wait that timed out as EM_ERR_HIP);
  * row_radix / col_radix: radix sequence of the interior STEM row pass and
- * of the propagator column pass (0 = the size's default; at 1024: 1 =
- * 16.4.16, 2 = 4.16.16 against 16.16.4, and 3 = rows 16.16.4 / columns
- * 16.4.16 with their per-lane twiddles in factored FMA form; at 2048: 1 =
- * 16.8.16, 2 = 8.16.16 against 16.16.8; other sizes have only 0). */
+ * of the propagator column pass (at 1024: 0 = 16.16.4, 1 = 16.4.16 -- a new
+ * plan's choice for both passes --, 2 = 4.16.16, and 3 = rows 16.16.4 /
+ * columns 16.4.16 with their per-lane twiddles in factored FMA form; at
+ * 2048: 0 = 16.16.8, 1 = 16.8.16 -- a new plan's rows --, 2 = 8.16.16;
+ * other sizes have only 0). */
 typedef struct em_tuning {
   int row_blocks, col_blocks, col_tile, probe_group, col_variant, row_variant, persistent;
   int row_radix, col_radix;
