@@ -662,11 +662,12 @@ void col_pass_kernel(ColArgs a) {
     }
   }
 
-  // Wave loads and stores of the 12-value lines of 768, 1536 and 3072 through
-  // a buffer per probe plane: one per-lane byte offset, the row step of each
-  // slot in a scalar register, instead of a 64-bit address per slot.  Their
-  // probe-slices/s rose 8%, 2% and 4%; the same form at 384, 1024, 2048 and
-  // 4096 moved them by -5% to +0.3%, so those keep global addresses.
+  // Wave loads and stores (and the late P loads) of the 12-value lines of 768,
+  // 1536 and 3072 through a buffer per plane: one per-lane byte offset, the
+  // row step of each slot in a scalar register, instead of a 64-bit address
+  // per slot.  Their probe-slices/s rose 8%, 3% and 6%; the same form at 384,
+  // 1024, 2048 and 4096 moved them by -5% to +0.3%, so those keep global
+  // addresses.
   constexpr bool CBUF = E == 12 && T >= 64;
   const std::uint32_t coff = (std::uint32_t)(((long long)tau * nx + x) * (long long)sizeof(float2));
   const std::uint32_t cstep = (std::uint32_t)((long long)T * nx * (long long)sizeof(float2));
@@ -726,7 +727,12 @@ void col_pass_kernel(ColArgs a) {
         if (zero_slot<E, ZK>(e)) {
           v[e] = make_float2(0.f, 0.f);
         } else {
-          float2 pe = Ptab[(long long)(tau + T * e) * nx + x];
+          float2 pe;
+          if constexpr (CBUF) {
+            pe = plane_load_s(plane_rsrc(Ptab, cplane), coff, (std::uint32_t)e * cstep);
+          } else {
+            pe = Ptab[(long long)(tau + T * e) * nx + x];
+          }
           if constexpr ((MODE & COL_CONJ_P) != 0) pe.y = -pe.y;
           v[e] = cmul(v[e], pe);
         }
