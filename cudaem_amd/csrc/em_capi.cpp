@@ -189,7 +189,6 @@ struct em_plan_s {
   DevBuf fine_tab, fine_az, fine_el, fine_r2, fine_q;
   DevBuf pk_ctr, pk_err;  // persistent slice loop: work queues and barrier count, error flag
   int pk_grid = 0;        // its workgroups (0: not yet queried)
-  DevBuf chunks;   // chunk tables of the ragged path
   DevBuf mchunks;  // chunk tables of em_multislice_forward_multi
   int vstore_slices = -1;
   hipStream_t gstream = nullptr;  // non-blocking stream for graph capture
@@ -201,6 +200,17 @@ struct em_plan_s {
   PinnedBuf pos_host;
   DevBuf pos_dev;
   hipEvent_t pos_copied = nullptr, stem_in = nullptr, stem_out = nullptr;
+  // em_multislice_ragged (on the plan of request 0): instantiated graphs of
+  // request sets kept across calls, each with its chunk table, keyed by
+  // everything a capture bakes in; and the events that order the graph
+  // stream after the caller's stream and back
+  struct RaggedGraph {
+    std::string key;
+    Graph graph;
+    DevBuf chunks;
+  };
+  std::vector<std::unique_ptr<RaggedGraph>> ragged_graphs;
+  hipEvent_t rag_in = nullptr, rag_out = nullptr;
   // per-launch timing of the STEM slice loop's row / column passes: event
   // pairs (start, end) of each timed launch, kind 0 = row, 1 = column
   bool pass_timing = false;
@@ -212,10 +222,12 @@ struct em_plan_s {
     tkind.clear();
   }
   ~em_plan_s() {
-    if (gstream) (void)hipStreamSynchronize(gstream);
+    // a ragged graph kept on another plan may still read this plan's tables
+    (void)hipDeviceSynchronize();
     clear_pass_events();
     stem_graphs.clear();
-    for (hipEvent_t e : {pos_copied, stem_in, stem_out})
+    ragged_graphs.clear();
+    for (hipEvent_t e : {pos_copied, stem_in, stem_out, rag_in, rag_out})
       if (e) (void)hipEventDestroy(e);
     if (gstream) (void)hipStreamDestroy(gstream);
   }
@@ -663,7 +675,7 @@ em_status em_plan_create(const em_grid* grid, const em_beam* beam, int max_batch
   EM_HIP(p.get(), device_scratch(p->device, &p->scratch));
   EM_HIP(p.get(), prepare_col_kernels(grid->ny));
   EM_HIP(p.get(), hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
-  for (hipEvent_t* e : {&p->pos_copied, &p->stem_in, &p->stem_out})
+  for (hipEvent_t* e : {&p->pos_copied, &p->stem_in, &p->stem_out, &p->rag_in, &p->rag_out})
     EM_HIP(p.get(), hipEventCreateWithFlags(e, hipEventDisableTiming));
   EM_TRY(upload_twiddles(p.get(), p->tw_x, grid->nx));
   EM_TRY(upload_twiddles(p.get(), p->tw_y, grid->ny));
@@ -1836,85 +1848,123 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
     for (const Op& op : oc)
       if (op.kind != OP_DET && op.count > 65535) return p0->fail(EM_ERR_INVALID, "too many probes in one pass");
 
-  // size and fill every buffer before capture
+  // size every buffer before capture (and before the key reads its address)
   for (int r = 0; r < nreq; ++r) {
     em_plan p = plans[r];
     const int ntiles = col_ntiles(p->grid.nx, p->grid.ny);
     EM_HIP(p, p->dpart.ensure(sizeof(float) * (size_t)nprobe[r] * ndet * ntiles));
   }
-  EM_HIP(p0, p0->chunks.ensure(sizeof(PassChunk) * chunks.size()));
-  EM_HIP(p0, hipMemcpy(p0->chunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
-  const PassChunk* dchunks = p0->chunks.as<PassChunk>();
 
-  EventSet evs;
-  hipEvent_t fork = nullptr, join = nullptr;
-  std::vector<hipEvent_t> done(cls.size(), nullptr);
-  EM_HIP(p0, evs.make(&fork, hipEventDisableTiming));
-  EM_HIP(p0, evs.make(&join, hipEventDisableTiming));
-  for (auto& e : done) EM_HIP(p0, evs.make(&e, hipEventDisableTiming));
-  EM_HIP(p0, hipEventRecord(fork, user));
-  EM_HIP(p0, hipStreamWaitEvent(p0->gstream, fork, 0));
-  Graph graph;
-  em_status st = p0->hip(hipStreamBeginCapture(p0->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
-  hipEvent_t cap_fork = nullptr;
-  if (st == EM_OK) st = p0->hip(evs.make(&cap_fork, hipEventDisableTiming), "event");
-  if (st == EM_OK) st = p0->hip(hipEventRecord(cap_fork, p0->gstream), "record");
-  for (size_t c = 0; c < cls.size() && st == EM_OK; ++c) {
-    // each class is a branch of the graph on its first plan's private stream
-    // (class 0 holds request 0)
-    em_plan pc = plans[cls[c][0]];
-    hipStream_t br = pc->gstream;
-    if (c > 0) st = pc->hip(hipStreamWaitEvent(br, cap_fork, 0), "fork");
-    for (size_t k = 0; k < ops[c].size() && st == EM_OK; ++k) {
-      const Op& op = ops[c][k];
-      if (op.kind == OP_ROW) {
-        RowArgs ra = row_args(pc, nullptr, nullptr, op.count);
-        ra.pb = 1;
-        ra.chunks = dchunks + op.first;
-        ra.skip_in = op.skip_in;
-        ra.skip_out = op.skip_out;
-        ra.zk = (op.skip_in && op.skip_out) ? pc->zk_x : 0;
-        st = pc->hip(launch_row(op.mode, ra, br), "row pass");
-      } else if (op.kind == OP_COL) {
-        ColArgs cc = col_args(pc, nullptr, nullptr, op.count);
-        cc.pb = 1;
-        cc.chunks = dchunks + op.first;
-        cc.cols = 0;
-        cc.variant = COL_VARIANT_BLOCK;
-        cc.skip_lo = pc->skip_lo;
-        cc.skip_hi = pc->skip_hi;
-        cc.zk = pc->zk_y;
-        st = pc->hip(launch_col(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS, cc, br),
-                     "column pass");
-      } else {
-        const int r = op.req;
-        em_plan p = plans[r];
-        const int B = nprobe[r];
-        ColArgs cd = col_args(p, reinterpret_cast<float2*>(psi[r]), nullptr, B);
-        cd.ndet = ndet;
-        for (int d = 0; d < ndet; ++d) {
-          cd.dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
-          cd.dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
-        }
-        cd.dpart = p->dpart.as<float>();
-        st = p->hip(launch_col(COL_PRE_FFT | COL_DET, cd, br), "column pass");
-        if (st == EM_OK)
-          st = p->hip(launch_det_reduce(B, ndet, col_ntiles(p->grid.nx, p->grid.ny), p->dpart.as<float>(), 1.f, 0,
-                                        out[r] + (long long)op.io * ndet * B, B, br),
-                      "detector reduce");
-      }
-    }
-    if (st == EM_OK) st = pc->hip(hipEventRecord(done[c], br), "record");
-    if (st == EM_OK && c > 0) st = pc->hip(hipStreamWaitEvent(p0->gstream, done[c], 0), "join");
+  // Everything a capture bakes in: the requests' buffers and slice counts, the
+  // detector bands and stride, each plan's launch shapes, skip ranges and
+  // tables, and the chunk table built from them.
+  std::string key;
+  auto put = [&key](const void* d, size_t n) { key.append(reinterpret_cast<const char*>(d), n); };
+  put(&nreq, sizeof(nreq));
+  put(&ndet, sizeof(ndet));
+  put(&stride, sizeof(stride));
+  put(inner, sizeof(double) * ndet);
+  put(outer, sizeof(double) * ndet);
+  for (int r = 0; r < nreq; ++r) {
+    const em_plan p = plans[r];
+    const void* ptrs[9] = {p, V[r], psi[r], out[r], p->dpart.p, p->P.p, p->tw_x.p, p->tw_y.p, p->twt_x.p};
+    put(ptrs, sizeof(ptrs));
+    put(&p->twt_y.p, sizeof(void*));
+    put(&S[r], sizeof(int));
+    put(&nprobe[r], sizeof(int));
+    put(&p->tune, sizeof(em_tuning));
+    const int sk[6] = {p->skip_lo, p->skip_hi, p->skip_lo_y, p->skip_hi_y, p->zk_x, p->zk_y};
+    put(sk, sizeof(sk));
+    put(&p->k.lambda, sizeof(double));
   }
-  const hipError_t ec = hipStreamEndCapture(p0->gstream, &graph.g);
-  if (st == EM_OK) st = p0->hip(ec, "end capture");
-  if (st == EM_OK) st = p0->hip(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0), "instantiate");
-  if (st == EM_OK) st = p0->hip(hipGraphLaunch(graph.x, p0->gstream), "graph launch");
-  if (st == EM_OK) st = p0->hip(hipEventRecord(join, p0->gstream), "record");
-  if (st == EM_OK) st = p0->hip(hipStreamWaitEvent(user, join, 0), "join");
-  if (st == EM_OK) st = p0->hip(hipStreamSynchronize(p0->gstream), "sync");
-  return st;
+  put(chunks.data(), sizeof(PassChunk) * chunks.size());
+
+  em_plan_s::RaggedGraph* rg = nullptr;
+  for (auto& e : p0->ragged_graphs)
+    if (e->key == key) rg = e.get();
+  if (!rg) {
+    auto entry = std::make_unique<em_plan_s::RaggedGraph>();
+    entry->key = key;
+    EM_HIP(p0, entry->chunks.ensure(sizeof(PassChunk) * chunks.size()));
+    EM_HIP(p0, hipMemcpy(entry->chunks.p, chunks.data(), sizeof(PassChunk) * chunks.size(), hipMemcpyHostToDevice));
+    const PassChunk* dchunks = entry->chunks.as<PassChunk>();
+
+    EventSet evs;
+    std::vector<hipEvent_t> done(cls.size(), nullptr);
+    for (auto& e : done) EM_HIP(p0, evs.make(&e, hipEventDisableTiming));
+    em_status st = p0->hip(hipStreamBeginCapture(p0->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipEvent_t cap_fork = nullptr;
+    if (st == EM_OK) st = p0->hip(evs.make(&cap_fork, hipEventDisableTiming), "event");
+    if (st == EM_OK) st = p0->hip(hipEventRecord(cap_fork, p0->gstream), "record");
+    for (size_t c = 0; c < cls.size() && st == EM_OK; ++c) {
+      // each class is a branch of the graph on its first plan's private stream
+      // (class 0 holds request 0)
+      em_plan pc = plans[cls[c][0]];
+      hipStream_t br = pc->gstream;
+      if (c > 0) st = pc->hip(hipStreamWaitEvent(br, cap_fork, 0), "fork");
+      for (size_t k = 0; k < ops[c].size() && st == EM_OK; ++k) {
+        const Op& op = ops[c][k];
+        if (op.kind == OP_ROW) {
+          RowArgs ra = row_args(pc, nullptr, nullptr, op.count);
+          ra.pb = 1;
+          ra.chunks = dchunks + op.first;
+          ra.skip_in = op.skip_in;
+          ra.skip_out = op.skip_out;
+          ra.zk = (op.skip_in && op.skip_out) ? pc->zk_x : 0;
+          st = pc->hip(launch_row(op.mode, ra, br), "row pass");
+        } else if (op.kind == OP_COL) {
+          ColArgs cc = col_args(pc, nullptr, nullptr, op.count);
+          cc.pb = 1;
+          cc.chunks = dchunks + op.first;
+          cc.cols = 0;
+          cc.variant = COL_VARIANT_BLOCK;
+          cc.skip_lo = pc->skip_lo;
+          cc.skip_hi = pc->skip_hi;
+          cc.zk = pc->zk_y;
+          st = pc->hip(launch_col(COL_PRE_FFT | COL_MUL_P | COL_POST_IFFT | COL_STORE | COL_CHUNKS, cc, br),
+                       "column pass");
+        } else {
+          const int r = op.req;
+          em_plan p = plans[r];
+          const int B = nprobe[r];
+          ColArgs cd = col_args(p, reinterpret_cast<float2*>(psi[r]), nullptr, B);
+          cd.ndet = ndet;
+          for (int d = 0; d < ndet; ++d) {
+            cd.dg2min[d] = mrad_to_g2(inner[d], p->k.lambda, false);
+            cd.dg2max[d] = mrad_to_g2(outer[d], p->k.lambda, true);
+          }
+          cd.dpart = p->dpart.as<float>();
+          st = p->hip(launch_col(COL_PRE_FFT | COL_DET, cd, br), "column pass");
+          if (st == EM_OK)
+            st = p->hip(launch_det_reduce(B, ndet, col_ntiles(p->grid.nx, p->grid.ny), p->dpart.as<float>(), 1.f, 0,
+                                          out[r] + (long long)op.io * ndet * B, B, br),
+                        "detector reduce");
+        }
+      }
+      if (st == EM_OK) st = pc->hip(hipEventRecord(done[c], br), "record");
+      if (st == EM_OK && c > 0) st = pc->hip(hipStreamWaitEvent(p0->gstream, done[c], 0), "join");
+    }
+    const hipError_t ec = hipStreamEndCapture(p0->gstream, &entry->graph.g);
+    if (st == EM_OK) st = p0->hip(ec, "end capture");
+    if (st == EM_OK) st = p0->hip(hipGraphInstantiate(&entry->graph.x, entry->graph.g, nullptr, nullptr, 0), "instantiate");
+    if (st != EM_OK) return st;
+    if (p0->ragged_graphs.size() >= 4) {
+      // the oldest graph may still run on the graph stream
+      EM_HIP(p0, hipStreamSynchronize(p0->gstream));
+      p0->ragged_graphs.erase(p0->ragged_graphs.begin());
+    }
+    rg = entry.get();
+    p0->ragged_graphs.push_back(std::move(entry));
+  }
+
+  // the graph stream runs after the caller's stream, and the caller's stream
+  // after it: no host synchronisation
+  EM_HIP(p0, hipEventRecord(p0->rag_in, user));
+  EM_HIP(p0, hipStreamWaitEvent(p0->gstream, p0->rag_in, 0));
+  EM_HIP(p0, hipGraphLaunch(rg->graph.x, p0->gstream));
+  EM_HIP(p0, hipEventRecord(p0->rag_out, p0->gstream));
+  EM_HIP(p0, hipStreamWaitEvent(user, p0->rag_out, 0));
+  return EM_OK;
 }
 
 }  // extern "C"

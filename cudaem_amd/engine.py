@@ -464,25 +464,34 @@ class Plan:
         return frames[:nf.value]
 
 
-def ragged(plans, Vs, psis, inner_mrad, outer_mrad, stride):
+def ragged(plans, Vs, psis, inner_mrad, outer_mrad, stride, outs=None):
     """One graph launch over requests on different grids (em_multislice_ragged).
 
     Returns per request a tensor [n_out, ndet, B] of detector sums after every
-    `stride` slices and after the last one."""
+    `stride` slices and after the last one: `outs` when given (reused, so a
+    repeated request set re-launches the graph kept on plans[0]), else new
+    tensors."""
     torch = _torch()
     n = len(plans)
     if not (len(Vs) == n == len(psis)) or n == 0:
         raise ValueError("one V and one psi per plan")
     if len({id(p) for p in plans}) != n:
         raise ValueError("requests need distinct plans")
+    if outs is not None and len(outs) != n:
+        raise ValueError("one output per plan")
     inner, outer = Plan._angles(inner_mrad, outer_mrad)
-    outs = []
-    for p, V, psi in zip(plans, Vs, psis):
+    new_outs = []
+    for i, (p, V, psi) in enumerate(zip(plans, Vs, psis)):
         S, B = V.shape[0], psi.shape[0]
         _need(V, (S, p.ny, p.nx), torch.float32, "V")
         _need(psi, (B, p.ny, p.nx), torch.complex64, "psi")
         nout = (S + stride - 1) // stride if S % stride == 0 else S // stride + 1
-        outs.append(torch.zeros((nout, len(inner), B), dtype=torch.float32, device="cuda"))
+        if outs is None:
+            new_outs.append(torch.zeros((nout, len(inner), B), dtype=torch.float32, device="cuda"))
+        else:
+            _need(outs[i], (nout, len(inner), B), torch.float32, "outs[%d]" % i)
+            new_outs.append(outs[i])
+    outs = new_outs
     arrP = (ctypes.c_void_p * n)(*[p.handle for p in plans])
     arrV = (ctypes.c_void_p * n)(*[V.data_ptr() for V in Vs])
     arrPsi = (ctypes.c_void_p * n)(*[q.data_ptr() for q in psis])

@@ -362,7 +362,10 @@ em_status em_thickness_multi(em_plan plan, const float* const* V_dev, int S, voi
  * plan plans[r] (distinct plans), psi_dev[r] incident waves [B_r][N_r]
  * (overwritten), S[r] slices of V_dev[r], detector sums every `stride`
  * slices and after the last one into out_dev[r][nout_r][ndet][B_r] where
- * nout_r = ceil(S_r/stride). */
+ * nout_r = ceil(S_r/stride).  The work is ordered on hip_stream and the call
+ * returns without waiting for it.  plans[0] keeps the instantiated graph (up
+ * to four request sets): a call with the same buffers, slice counts,
+ * detector bands, stride and plan settings re-launches it. */
 em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V_dev, const int* S,
                                void* const* psi_dev, const int* nprobe, int ndet,
                                const double* inner_mrad, const double* outer_mrad, int stride,

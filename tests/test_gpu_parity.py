@@ -532,6 +532,58 @@ def test_ragged_matches_separate_runs(em, mix):
         p.close()
 
 
+def test_ragged_graph_kept_across_calls(em):
+    """A repeated ragged request set re-launches the graph the first call
+    captured: with the inputs restored and the outputs reused, a second call
+    gives bit-identical detector frames and waves, and a third in between
+    with other detector bands (a new graph) gives those bands' frames, equal
+    to a fresh set of plans."""
+    import torch
+    import cudaem_amd.specimen as sp
+    at = sp.si_diamond(3, 3, 6)
+    mix = ((128, 300.0, 2), (256, 200.0, 3), (128, 200.0, 1))
+
+    def build():
+        plans, Vs, psis = [], [], []
+        for n, e0, B in mix:
+            p = em.Plan(n, n, at["lx"], at["ly"], 1.35775, e0, max_batch=B)
+            p.set_specimen(at["x"], at["y"], at["z"], at["Z"], at["occ"])
+            Vs.append(p.potential_slices())
+            xs = np.linspace(1, at["lx"] - 1, B)
+            psis.append(p.build_probes(probe_lens(15.0), xs, xs[::-1].copy()))
+            plans.append(p)
+        return plans, Vs, psis
+
+    plans, Vs, psis = build()
+    psi0 = [q.clone() for q in psis]
+    outs = em.ragged(plans, Vs, psis, [10.0], [40.0], stride=3)
+    first = [_np(o) for o in outs]
+    first_psi = [_np(q) for q in psis]
+    for q, q0 in zip(psis, psi0):
+        q.copy_(q0)
+    other = [_np(o) for o in em.ragged(plans, Vs, psis, [20.0], [60.0], stride=3)]
+    for q, q0 in zip(psis, psi0):
+        q.copy_(q0)
+    em.ragged(plans, Vs, psis, [10.0], [40.0], stride=3, outs=outs)
+    again = [_np(o) for o in outs]
+    again_psi = [_np(q) for q in psis]
+    torch.cuda.synchronize()
+    for p in plans:
+        p.close()
+    for x, y in zip(first, again):
+        assert np.array_equal(x, y)
+    for x, y in zip(first_psi, again_psi):
+        assert np.array_equal(x, y)
+    plans, Vs, psis = build()
+    fresh = [_np(o) for o in em.ragged(plans, Vs, psis, [20.0], [60.0], stride=3)]
+    torch.cuda.synchronize()
+    for p in plans:
+        p.close()
+    for x, y in zip(other, fresh):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(other[0], first[0])
+
+
 def test_forward_multi_matches_per_probe_runs(em):
     """em_multislice_forward_multi with a potential stack per probe (stacks of
     different depth; probes 1 and 2 share one stack, probes 3 and 4 share
