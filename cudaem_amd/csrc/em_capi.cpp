@@ -1856,8 +1856,8 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
   }
 
   // Everything a capture bakes in: the requests' buffers and slice counts, the
-  // detector bands and stride, each plan's launch shapes, skip ranges and
-  // tables, and the chunk table built from them.
+  // detector bands and stride, each plan's grid, wavelength, launch shapes,
+  // skip ranges and tables, and the chunk table built from them.
   std::string key;
   auto put = [&key](const void* d, size_t n) { key.append(reinterpret_cast<const char*>(d), n); };
   put(&nreq, sizeof(nreq));
@@ -1876,6 +1876,10 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
     const int sk[6] = {p->skip_lo, p->skip_hi, p->skip_lo_y, p->skip_hi_y, p->zk_x, p->zk_y};
     put(sk, sizeof(sk));
     put(&p->k.lambda, sizeof(double));
+    const int gi[3] = {p->grid.nx, p->grid.ny, p->grid.bwl};
+    const double gd[3] = {p->grid.lx, p->grid.ly, p->grid.dz};
+    put(gi, sizeof(gi));
+    put(gd, sizeof(gd));
   }
   put(chunks.data(), sizeof(PassChunk) * chunks.size());
 
