@@ -1789,6 +1789,11 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
         }
       return std::make_pair(first, (int)chunks.size() - first);
     };
+    // a class whose one chunk of row workgroups fills the resident slots
+    // (2048 and up) takes all its probes per workgroup: each row's
+    // transmission is then formed once per slice step (ragged mix: a 2048^2
+    // request of 2 probes, +4%)
+    const bool one_row_chunk = pc->grid.ny / row_lines_per_block(pc->grid.nx) >= row_slots(pc);
     for (int s = 0; s < Smax; ++s) {
       const bool rec_step = (s + 1) % stride == 0;
       // row pass: one launch per output-skip setting (a request's last slice
@@ -1804,7 +1809,7 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
           btot += nprobe[r];
         }
         if (reqs.empty()) continue;
-        const auto fc = add_chunks(reqs, row_pb(pc, btot), true, s);
+        const auto fc = add_chunks(reqs, one_row_chunk ? btot : row_pb(pc, btot), true, s);
         Op op{};
         op.kind = OP_ROW;
         op.first = fc.first;
