@@ -1839,7 +1839,11 @@ em_status em_multislice_ragged(int nreq, em_plan* plans, const float* const* V, 
       if (!reqs.empty()) {
         const int blocks = pc->grid.nx / col_tile(pc->grid.ny);
         const int target = std::max(1, pc->tune.col_blocks);
-        const int groups = std::max(1, std::min(btot, (target + blocks - 1) / blocks));
+        // a class of at most two probes on such large grids: one chunk's
+        // two-probe workgroups take as many rounds as two one-probe chunks
+        // (a column workgroup holds a CU) and load P once (+1.6% on the mix)
+        const int groups = (one_row_chunk && btot <= 2) ? 1
+                                                        : std::max(1, std::min(btot, (target + blocks - 1) / blocks));
         const auto fc = add_chunks(reqs, (btot + groups - 1) / groups, false, s);
         Op op{};
         op.kind = OP_COL;
